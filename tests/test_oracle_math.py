@@ -1,0 +1,281 @@
+"""Pin the oracle with what the mathematics fixes, independently of the
+reference's operation order: storage rounding against numpy, BGK collision
+against an independent numpy D2Q9 implementation, streaming as a
+permutation, conservation laws, equilibrium fixed point, and closed-form
+flows (Taylor-Green decay, Poiseuille profile, Ghia cavity, cylinder
+symmetry)."""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+# D2Q9 in the oracle's order 01 03 05 08 10 12 14 17 19
+CX = np.array([1, 1, 1, 0, -1, -1, -1, 0, 0])
+CY = np.array([-1, 1, 0, 1, -1, 1, 0, -1, 0])
+W = np.array([1 / 36, 1 / 36, 1 / 9, 1 / 9, 1 / 36, 1 / 36, 1 / 9, 1 / 9, 4 / 9])
+OPP = np.array([5, 4, 6, 7, 1, 0, 2, 3, 8])
+
+
+def feq_shifted(rho, ux, uy):
+    """textbook D2Q9 equilibrium minus the weights, shape (9, ...)"""
+    cu = CX[:, None, None] * ux + CY[:, None, None] * uy
+    usq = ux * ux + uy * uy
+    return W[:, None, None] * rho * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * usq) - W[:, None, None]
+
+
+def moments(f):
+    rho = f.sum(0) + 1.0
+    ux = (CX[:, None, None] * f).sum(0) / rho
+    uy = (CY[:, None, None] * f).sum(0) / rho
+    return rho, ux, uy
+
+
+# ----------------------------------------------------------- rounding
+
+def test_f16_decode_exhaustive():
+    L = po.lib()
+    bits = np.arange(65536, dtype=np.uint16)
+    ref = bits.view(np.float16).astype(np.float32)
+    got = np.array([L.orc_f16_to_f32(int(b)) for b in bits], dtype=np.float32)
+    nan = np.isnan(ref)
+    assert np.array_equal(np.isnan(got), nan)
+    assert np.array_equal(got[~nan].view(np.uint32), ref[~nan].view(np.uint32))
+
+
+def test_f16_encode_against_numpy():
+    L = po.lib()
+    rng = np.random.default_rng(7)
+    vals = np.concatenate([
+        rng.standard_normal(4000).astype(np.float32) * np.float32(1e-3),
+        (rng.random(2000) * 7e4).astype(np.float32),
+        (rng.standard_normal(2000) * 1e-6).astype(np.float32),   # subnormal halves
+        np.array([0.0, -0.0, 65504, 65519.99, 65520, 65536, 6.1035156e-05, 5.96e-08,
+                  2.98e-08, 2.99e-08, np.inf, -np.inf, 2147483647.0], dtype=np.float32),
+        # exact ties between adjacent halves
+        (np.arange(1, 200, dtype=np.float32) * np.float32(2 ** -11) + np.float32(1.0)),
+    ])
+    with np.errstate(over="ignore"):
+        ref = vals.astype(np.float16).view(np.uint16)
+    got = np.array([L.orc_f32_to_f16(float(v)) for v in vals], dtype=np.uint16)
+    assert np.array_equal(got, ref)
+
+
+def test_bf16_round_nearest_even():
+    L = po.lib()
+    rng = np.random.default_rng(3)
+    vals = np.concatenate([rng.standard_normal(5000).astype(np.float32) * np.float32(1e-2),
+                           # ties: low 16 bits exactly 0x8000
+                           (rng.integers(0x3c000000, 0x40000000, 500, dtype=np.uint32)
+                            & np.uint32(0xFFFF0000) | np.uint32(0x8000)).view(np.float32)])
+    got = np.array([L.orc_f32_to_bf16(float(v)) for v in vals], dtype=np.uint16)
+    # independent rule: pick the nearer of the two neighbouring bf16 values
+    b = vals.view(np.uint32)
+    lo = (b >> 16).astype(np.uint16)
+    hi = lo + np.uint16(1)
+    flo = (lo.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    fhi = (hi.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    x = vals.astype(np.float64)
+    dlo, dhi = np.abs(x - flo), np.abs(fhi - x)
+    pick_hi = (dhi < dlo) | ((dhi == dlo) & (lo % 2 == 1))
+    ref = np.where(pick_hi, hi, lo)
+    assert np.array_equal(got, ref)
+    back = np.array([L.orc_bf16_to_f32(int(g)) for g in got[:10]], dtype=np.float32)
+    assert np.array_equal(back.view(np.uint32), got[:10].astype(np.uint32) << 16)
+
+
+# ----------------------------------------------------------- collision
+
+def _random_state(o, amp=0.02, seed=0):
+    rng = np.random.default_rng(seed)
+    m2, l2 = o.plane_shape
+    rho = 1 + amp * rng.standard_normal((m2, l2))
+    ux = amp * rng.standard_normal((m2, l2))
+    uy = amp * rng.standard_normal((m2, l2))
+    f = feq_shifted(rho, ux, uy) + 1e-3 * amp * rng.standard_normal((9, m2, l2))
+    o.set_raw(f)
+    return f
+
+
+def _pull_periodic(f):
+    """streaming on a periodic interior: f_k(x) <- f_k(x - c_k)"""
+    g = np.empty_like(f)
+    for k in range(9):
+        g[k] = np.roll(np.roll(f[k], CX[k], axis=1), CY[k], axis=0)
+    return g
+
+
+def test_collision_matches_textbook_bgk():
+    """one FUSED periodic step = pull + BGK relaxation to the textbook
+    equilibrium; checked on the interior with an independent numpy code"""
+    l, m = 12, 10
+    o = po.Oracle(l, m, bc="periodic", impl="fused", prec="double", svisc=0.1, tgv=0)
+    f0 = _random_state(o)
+    omega = o.constants()["omega"]
+    assert omega == pytest.approx(2 / (6 * 0.1 + 1), rel=1e-15)
+    inner = f0[:, 1:-1, 1:-1]
+    pulled = _pull_periodic(inner)
+    rho, ux, uy = moments(pulled)
+    expect = pulled - omega * (pulled - feq_shifted(rho, ux, uy))
+    o.step(1)
+    got = o.pops()[:, 1:-1, 1:-1]
+    np.testing.assert_allclose(got, expect, rtol=0, atol=2e-15)
+
+
+def test_omega_one_gives_equilibrium():
+    l, m = 10, 8
+    o = po.Oracle(l, m, bc="periodic", impl="original", prec="double", svisc=1 / 6, tgv=0)
+    assert o.constants()["omega"] == 1.0
+    f0 = _random_state(o, seed=5)
+    pulled = _pull_periodic(f0[:, 1:-1, 1:-1])
+    o.step(1)
+    np.testing.assert_allclose(o.pops()[:, 1:-1, 1:-1], feq_shifted(*moments(pulled)),
+                               rtol=0, atol=2e-15)
+
+
+def test_channel_force_adds_momentum_not_mass():
+    """CHANNEL forcing adds 6*fgrad*rho of x-momentum per node and no mass
+    (SRC/col.F90:212-237); checked on a uniform rest state."""
+    l, m = 8, 6
+    o = po.Oracle(l, m, bc="channel", impl="fused", prec="double", svisc=0.1, u0=0.05)
+    fgrad = o.constants()["fgrad"]
+    assert fgrad == pytest.approx(4 * 0.05 * 0.1 / (3 * m * m), rel=1e-15)
+    o.step(1)
+    f = o.pops()[:, 2:-2, 2:-2]  # away from the walls
+    rho, ux, uy = moments(f)
+    np.testing.assert_allclose(rho, 1.0, atol=1e-15)
+    np.testing.assert_allclose(ux * rho, 6 * fgrad, rtol=1e-12)
+
+
+def test_equilibrium_is_fixed_point():
+    l, m = 10, 8
+    o = po.Oracle(l, m, bc="periodic", impl="fused", prec="double", svisc=0.07, tgv=0)
+    shape = o.plane_shape
+    f = feq_shifted(np.full(shape, 1.01), np.full(shape, 0.03), np.full(shape, -0.02))
+    o.set_raw(f)
+    o.step(5)
+    np.testing.assert_allclose(o.pops()[:, 1:-1, 1:-1], f[:, 1:-1, 1:-1], atol=2e-16)
+
+
+def test_movef_is_permutation():
+    """ORIGINAL streaming is an exact copy: after boundaries+propagation the
+    b set equals the pulled a set bit for bit (tiny grid brute force)."""
+    l, m = 6, 5
+    o = po.Oracle(l, m, bc="periodic", impl="original", prec="single", tgv=0)
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((9,) + o.plane_shape).astype(np.float32)
+    o.set_raw(a)
+    o.boundaries()
+    o.propagation()
+    a_bc = o.raw("a")
+    b = o.raw("b")
+    for k in range(8):
+        for j in range(1, m + 1):
+            for i in range(1, l + 1):
+                assert b[k, j, i] == a_bc[k, j - CY[k], i - CX[k]]
+        # periodic halos are exact copies of the opposite interior edge
+        rolled = np.roll(np.roll(a[k, 1:-1, 1:-1], CX[k], axis=1), CY[k], axis=0)
+        assert np.array_equal(rolled, b[k, 1:-1, 1:-1])
+
+
+@pytest.mark.parametrize("bc", ["driven", "periodic", "channel", "inflow", "couette", "step"])
+@pytest.mark.parametrize("prec", ["single", "double", "mixed1", "bf16"])
+def test_original_equals_fused(bc, prec):
+    """both implementations perform the same arithmetic (SRC/col.F90 vs
+    SRC/col_MC.F90); FUSED+step differs only inside the solid step, where
+    the reference reads never-written halos of the second buffer."""
+    kw = dict(bc=bc, prec=prec, svisc=0.08, u0=0.05)
+    a = po.Oracle(40, 20, impl="original", **kw)
+    b = po.Oracle(40, 20, impl="fused", **kw)
+    a.step(30)
+    b.step(30)
+    ra, rb = a.raw()[:, 1:-1, 1:-1], b.raw()[:, 1:-1, 1:-1]
+    if bc == "step":
+        stepx, stepy = 40 // 20, 20 // 2
+        ra, rb = ra[:, stepy:, :], rb[:, stepy:, :]  # rows above the step
+    assert np.array_equal(ra, rb)
+
+
+# ---------------------------------------------------------- conservation
+
+@pytest.mark.parametrize("bc", ["periodic", "driven", "couette"])
+def test_mass_conserved(bc):
+    o = po.Oracle(24, 20, bc=bc, impl="fused", prec="double", svisc=0.05, u0=0.1)
+    m0 = o.pops()[:, 1:-1, 1:-1].sum()
+    o.step(200)
+    m1 = o.pops()[:, 1:-1, 1:-1].sum()
+    assert abs(m1 - m0) < 1e-11
+
+
+def test_obstacle_mass_and_symmetry():
+    """uniform inflow past a vertically centred cylinder with periodic y:
+    the flow is mirror-symmetric about j = m/2 (u even, v odd)."""
+    l, m = 48, 24
+    o = po.Oracle(l, m, bc="inflow", obstacle=1, radius=4, impl="fused",
+                  prec="double", svisc=0.05)
+    o.step(120)
+    f = o.pops()
+    rho, ux, uy = moments(f[:, 1:-1, 1:-1])
+    j = np.arange(1, m + 1)
+    mirror = (m - j) % m  # j -> m - j on the periodic ring, 0-based row index
+    mirror_idx = np.where(mirror == 0, m, mirror) - 1
+    obs, bbox = o.obs()
+    fluid = obs == 0
+    # solid nodes hold values that depend on bcond_obs's serial j-outer
+    # order (SRC/bcond.obs.F90:50-59), which is not mirror symmetric
+    np.testing.assert_allclose(ux[fluid], ux[mirror_idx, :][fluid], atol=1e-12)
+    np.testing.assert_allclose(uy[fluid], -uy[mirror_idx, :][fluid], atol=1e-12)
+    assert obs.sum() > 0 and bbox[0] == 2 * l // 5 - 4
+
+
+# ---------------------------------------------------------- closed forms
+
+def test_taylor_green_decay():
+    """<u^2+v^2>(t) = (A^2/2) exp(-4 nu k^2 t), k = 2 pi / L
+    (TEST/TaylorGreenVortex/plot.gnu:171 uses 0.005 = 0.1^2/2)."""
+    L, nu, A, T = 48, 0.05, 0.01, 300
+    o = po.Oracle(L, L, bc="periodic", impl="fused", prec="double", svisc=nu, amplitude=A)
+    e0 = o.diagno()[3]
+    assert e0 == pytest.approx(A * A / 2, rel=1e-10)
+    k = 2 * np.pi / L
+    o.step(T)
+    e1 = o.diagno()[3]
+    # includes the start-up transient of the equilibrium initial state
+    assert e1 == pytest.approx(A * A / 2 * np.exp(-4 * nu * k * k * T), rel=1e-2)
+    o.step(T)
+    # decay rate over a later window: discretisation error O(k^2) only
+    assert o.diagno()[3] / e1 == pytest.approx(np.exp(-4 * nu * k * k * T), rel=5e-4)
+
+
+def test_poiseuille_profile():
+    """steady channel flow under the body force reaches u(y) = 4 u0 y'(1-y'),
+    y' = (j - 1/2)/m (TEST/PoiseuilleFlow/plot.gnu:172)"""
+    l, m, u0 = 4, 16, 0.01
+    o = po.Oracle(l, m, bc="channel", impl="fused", prec="double", svisc=1 / 6, u0=u0)
+    o.step(12000)
+    u, v, r = o.prof_j(l // 2)
+    y = (np.arange(1, m + 1) - 0.5) / m
+    np.testing.assert_allclose(u, 4 * u0 * y * (1 - y), rtol=0, atol=2e-3 * u0)
+    assert o.diagno()[3] == pytest.approx(8 * u0 * u0 / 15, rel=2e-2)
+
+
+@pytest.mark.slow
+def test_cavity_ghia_re100():
+    """Lid-driven cavity at Re = U N / nu = 100 against Ghia et al. 1982
+    (TEST/DrivenCavity/ghia.{x,y}.dat) on the centre lines."""
+    n, u0 = 64, 0.1
+    o = po.Oracle(n, n, bc="driven", impl="fused", prec="single", svisc=u0 * n / 100, u0=u0)
+    o.step(16000)
+    gy = np.loadtxt(os.path.join(HERE, "golden", "ghia_y.dat"))
+    gx = np.loadtxt(os.path.join(HERE, "golden", "ghia_x.dat"))
+    # vertical centre line x = 1/2 lies between columns n/2 and n/2+1
+    u = 0.5 * (o.prof_j(n // 2)[0] + o.prof_j(n // 2 + 1)[0]) / u0
+    y = (np.arange(1, n + 1) - 0.5) / n
+    v = 0.5 * (o.prof_i(n // 2)[1] + o.prof_i(n // 2 + 1)[1]) / u0
+    x = (np.arange(1, n + 1) - 0.5) / n
+    sel = (gy[:, 0] > 0.02) & (gy[:, 0] < 0.98)
+    assert np.max(np.abs(np.interp(gy[sel, 0], y, u) - gy[sel, 1])) < 0.03
+    sel = (gx[:, 0] > 0.02) & (gx[:, 0] < 0.98)
+    assert np.max(np.abs(np.interp(gx[sel, 0], x, v) - gx[sel, 1])) < 0.03
