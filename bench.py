@@ -1,0 +1,133 @@
+"""Benchmark: MLUPS of the FUSED FP32 step on one MI355X, its HBM roofline
+fraction, FP16-storage MLUPS, and the reference's CPU build timed on the
+host cores beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+One "step" is one lattice update of the whole 4096x4096 lid-driven cavity
+(Re = 10^4: svisc = 0.04096, u0 = 0.1), the single-solve bandwidth
+configuration (20,000 steps by default).  Populations are resident in HBM
+before the timed region; the timed region is bgk_run(K) bracketed by
+stream synchronisation.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "MLUPS (% HBM roofline) fp32 vs fp16-store; batched-sweep p50/p99 latency"
+BYTES_PER_LUP = {"single": 72, "fp16": 36, "bf16": 36, "double": 144, "mixed2": 72}
+
+
+def gpu_solve(n, prec, bc, steps, warmup, svisc, u0):
+    from bgk2d_gpu_amd import Solver
+    s = Solver(n, n, bc=bc, impl="fused", prec=prec, svisc=svisc, u0=u0,
+               scale=1024.0 if prec in ("fp16", "bf16") else 1.0, amplitude=0.01)
+    if warmup:
+        s.run(warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    s.run(steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    kernel_ms = s.last_run_ms()  # HIP events on the context stream, per step
+    ok = s.check(0.5)
+    diag = s.diagno()
+    s.close()
+    return dict(wall=wall, kernel_ms=kernel_ms, ok=ok, diag=diag.tolist())
+
+
+def cpu_baseline(budget_s=20.0):
+    """The reference's own FUSED SINGLE LDC sources built serially by flang
+    (oracle/_ref), on a bounded sample of the same 4096^2 workload."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_SINGLE_FUSED_LDC")
+    n = 4096
+    if os.path.exists(exe):
+        steps = 30
+        r = subprocess.run([exe, str(n), str(n), "0.04096", "0.1", str(steps), "16", "-"],
+                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                           cwd="/tmp", timeout=600)
+        m = re.search(r"mlups=\s*([0-9.]+)", r.stdout)
+        if m:
+            return {"value": float(m.group(1)), "unit": "MLUPS", "cores": 1, "kind": "reference",
+                    "sample": f"4096x4096 LDC Re=1e4, {steps} steps, serial flang -O3 build "
+                              "of the reference's FUSED SINGLE sources (oracle/_ref)"}
+    # fallback: the oracle (C restatement), one core
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    o = pyoracle.Oracle(n, n, bc="driven", impl="fused", prec="single", svisc=0.04096, u0=0.1)
+    steps = 20
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    return {"value": n * n * steps / dt / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
+            "sample": f"4096x4096 LDC Re=1e4, {steps} steps, oracle C restatement"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp16-steps", type=int, default=2000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(each request or slab shards per GPU; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    n = 4096
+    r32 = gpu_solve(n, "single", "driven", args.steps, args.warmup, 0.04096, 0.1)
+    mlups = n * n * args.steps / r32["wall"] / 1e6
+    kern_s = r32["kernel_ms"] / 1e3
+    achieved = BYTES_PER_LUP["single"] * n * n / kern_s / 1e9
+
+    # FP16 shifted storage: Taylor-Green 8192^2 (tau = 0.8, u0 = 0.01) vs FP32
+    n3 = 8192
+    t16 = gpu_solve(n3, "fp16", "periodic", args.fp16_steps, 50, 0.1, 0.0)
+    t32 = gpu_solve(n3, "single", "periodic", args.fp16_steps, 50, 0.1, 0.0)
+    ml16 = n3 * n3 * args.fp16_steps / t16["wall"] / 1e6
+    ml32 = n3 * n3 * args.fp16_steps / t32["wall"] / 1e6
+
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["wall"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "lid-driven cavity 4096x4096, FUSED, FP32 storage+compute, "
+                               "Re=1e4 (svisc=0.04096, u0=0.1), boundary folded into the pull",
+                   "global_batch": 1, "lattice": [n, n], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "kernel_ms_per_step": r32["kernel_ms"],
+        "healthy": bool(r32["ok"]),
+        "fp16_store": {"workload": "Taylor-Green 8192x8192 periodic, tau=0.8, u0=0.01, "
+                                   f"{args.fp16_steps} steps, scale=1024",
+                       "mlups_fp16": round(ml16, 1), "mlups_fp32": round(ml32, 1),
+                       "ratio": round(ml16 / ml32, 3),
+                       "hbm_frac_fp16": round(36 * n3 * n3 / (t16["kernel_ms"] / 1e3) / 1e9
+                                              / HBM_PEAK_GBS, 4)},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

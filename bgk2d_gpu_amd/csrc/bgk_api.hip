@@ -1,0 +1,818 @@
+// bgk_api.hip -- C ABI and runtime of libbgk2d_hip.so (see include/bgk2d.h).
+//
+// A context owns one HBM allocation (17 pitched population planes plus
+// the obstacle mask and reduction scratch), one HIP stream and the HIP
+// graphs it replays.  Host code derives the reference's constants in the
+// compute kind (hencol, build_bcond) and launches the kernels of
+// bgk_kernels.hpp in the reference's phase order.
+#include "../../include/bgk2d.h"
+#include "bgk_kernels.hpp"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace bgk;
+
+struct bgk_ctx {
+    bgk_params_t p;
+    int l, m;
+    int esize, A, off, pitch;
+    size_t plane_elems;  // elements per plane, incl. alignment padding
+    int dev;
+    hipStream_t s;
+    void *mem;
+    void *plane[17];  // [0..7] set 0, [8..15] set 1, [16] pop 19
+    int cur;          // which set is `a`
+    uint8_t *d_obs;
+    int imin, imax, jmin, jmax, nobs;
+    std::vector<int32_t> h_obs;
+    double *d_rows, *d_red, *h_red;
+    void *d_stage;
+    size_t stage_bytes;
+    // constants in the compute kind, widened
+    double omega, fgrad, u00, force, scale;
+    double inflow[3];
+    bool inited;
+    std::string err;
+    hipEvent_t ev0, ev1;
+    double last_ms;
+    std::map<long long, hipGraphExec_t> graphs;
+};
+
+namespace {
+
+int fail(bgk_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail((c), BGK_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+bool is_fp16(int prec) { return prec == BGK_PREC_FP16; }
+bool is_bf16(int prec) { return prec == BGK_PREC_BF16; }
+
+int esize_of(int prec)
+{
+    switch (prec) {
+    case BGK_PREC_DOUBLE: return 8;
+    case BGK_PREC_FP16:
+    case BGK_PREC_BF16: return 2;
+    default: return 4;
+    }
+}
+
+// host-side storage round trip used for the inflow constants
+float host_rt_f16(float v, float sc, float inv) { return (float)(_Float16)(v * sc) * inv; }
+float host_rt_bf16(float v, float sc, float inv)
+{
+    float t = v * sc;
+    uint32_t x;
+    memcpy(&x, &t, 4);
+    if ((x & 0x7fffffffu) > 0x7f800000u) {
+        x = ((x >> 16) | 0x40u) << 16;
+    } else {
+        x += 0x7fffu + ((x >> 16) & 1u);
+        x = (x >> 16) << 16;
+    }
+    float r;
+    memcpy(&r, &x, 4);
+    return r * inv;
+}
+
+// hencol + build_bcond in the compute kind C (SRC/hencol.F90:30-50,
+// SRC/build.bcond.F90:36-45), and the stored inflow equilibria of
+// SRC/bcond.inflow.F90:101-113 with u_inflow = 0.1 (a default-real literal).
+template <class C>
+void derive(bgk_ctx *c)
+{
+    const bgk_params_t &p = c->p;
+    C u0 = (C)p.u0, u00;
+    if (p.bc == BGK_BC_CHANNEL) {
+        u00 = (C)0.0;
+    } else {
+        u00 = u0;
+        u0 = (C)0.0;
+    }
+    C svisc = (C)p.svisc;
+    C omega = ((C)1.0 + (C)1.0) / (((C)6.0 * svisc) + (C)1.0);
+    if (is_fp16(p.prec) && p.mixed1_quirk) u0 = (C)0.0;
+    C fgrad = (C)4.0 * u0 * svisc / ((C)3.0 * (C)c->m * (C)c->m);
+    c->omega = (double)omega;
+    c->fgrad = (double)fgrad;
+    c->u00 = (double)u00;
+    c->force = (double)(u00 / (C)6.0);
+    const C xj = (C)0.1f, yj = (C)0.0;
+    const C cvsq = xj * xj + yj * yj;
+    const C cxq[3] = {eq_d<C>(xj - yj, cvsq), eq_d<C>(xj + yj, cvsq), eq_d<C>(xj, cvsq)};
+    const C wq[3] = {K<C>::p2, K<C>::p2, K<C>::p1};
+    for (int q = 0; q < 3; q++) {
+        C t = (C)1.0 * wq[q] * ((C)0.0 + cxq[q]);  // crho*p*(cte1+cx)
+        double r = (double)t;
+        if (is_fp16(p.prec)) r = host_rt_f16((float)t, (float)c->scale, (float)(1.0 / c->scale));
+        if (is_bf16(p.prec)) r = host_rt_bf16((float)t, (float)c->scale, (float)(1.0 / c->scale));
+        if (p.prec == BGK_PREC_MIXED2) r = (double)(float)t;
+        c->inflow[q] = r;
+    }
+}
+
+template <class C>
+Phys<C> phys(const bgk_ctx *c)
+{
+    Phys<C> P;
+    P.omega = (C)c->omega;
+    P.fgrad = (C)c->fgrad;
+    P.force = (C)c->force;
+    P.sc = (C)c->scale;
+    P.inv = (C)(1.0 / c->scale);
+    for (int q = 0; q < 3; q++) P.inflow[q] = (C)c->inflow[q];
+    P.channel_force = c->p.bc == BGK_BC_CHANNEL ? 1 : 0;
+    return P;
+}
+
+Lat lat(const bgk_ctx *c)
+{
+    Lat L;
+    const int a = c->cur, b = 1 - c->cur;
+    for (int k = 0; k < 8; k++) {
+        L.a[k] = c->plane[a * 8 + k];
+        L.b[k] = c->plane[b * 8 + k];
+    }
+    L.r = c->plane[16];
+    L.obs = c->d_obs;
+    L.l = c->l;
+    L.m = c->m;
+    L.pitch = c->pitch;
+    L.off = c->off;
+    L.imin = c->imin;
+    L.imax = c->imax;
+    L.jmin = c->jmin;
+    L.jmax = c->jmax;
+    L.stepx = c->l / 20;  // SRC/input.F90:74-75
+    L.stepy = c->m / 2;
+    return L;
+}
+
+dim3 grid2(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y); }
+
+const dim3 B2(64, 4);
+
+template <class S, class C>
+int launch_boundaries(bgk_ctx *c)
+{
+    Lat L = lat(c);
+    Phys<C> P = phys<C>(c);
+    const int l = c->l, m = c->m;
+    auto pass = [&](int ps, int t0, int n, int lid) {
+        if (n <= 0) return;
+        hipLaunchKernelGGL((k_bc<S, C>), dim3((n + 255) / 256), dim3(256), 0, c->s, L, P, ps, t0, n, lid);
+    };
+    switch (c->p.bc) {
+    case BGK_BC_PERIODIC:
+        pass(BP_XPER, 0, m + 2, 0);
+        pass(BP_YPER, 0, l + 2, 0);
+        break;
+    case BGK_BC_CHANNEL:
+        pass(BP_XPER, 0, m + 2, 0);
+        pass(BP_WALL_Y, 0, l + 2, 0);
+        break;
+    case BGK_BC_INFLOW:
+        pass(BP_INFLOW_X, 0, m + 2, 0);
+        pass(BP_YPER, 0, l + 2, 0);
+        break;
+    case BGK_BC_COUETTE:
+        pass(BP_XPER, 1, m, 0);
+        pass(BP_WALL_Y, 1, l, 1);
+        break;
+    case BGK_BC_STEP:
+        pass(BP_STEP_OUT, 0, m + 2, 0);
+        pass(BP_STEP_IN, L.stepy, m + 2 - L.stepy, 0);
+        pass(BP_WALL_Y, 1, l, 0);
+        pass(BP_STEP_TOP, 1, L.stepx, 0);
+        pass(BP_STEP_FACE, 1, L.stepy, 0);
+        break;
+    default:
+        pass(BP_DRV_X, 1, m, 0);
+        pass(BP_WALL_Y, 1, l, 1);
+        break;
+    }
+    if (c->p.obstacle && c->nobs > 0) {
+        int nx = c->imax - c->imin + 1, ny = c->jmax - c->jmin + 1;
+        hipLaunchKernelGGL((k_bc_obs<S>), grid2(nx, ny, dim3(16, 16)), dim3(16, 16), 0, c->s, L);
+    }
+    HIPCHK(c, hipGetLastError());
+    return BGK_OK;
+}
+
+template <class S, class C>
+int launch_propagation(bgk_ctx *c)
+{
+    if (c->p.impl != BGK_IMPL_ORIGINAL) return BGK_OK;
+    Lat L = lat(c);
+    hipLaunchKernelGGL((k_movef<S>), grid2(c->l, c->m, B2), B2, 0, c->s, L);
+    HIPCHK(c, hipGetLastError());
+    return BGK_OK;
+}
+
+template <class S, class C>
+int launch_collision(bgk_ctx *c)
+{
+    Lat L = lat(c);
+    Phys<C> P = phys<C>(c);
+    if (c->p.impl == BGK_IMPL_ORIGINAL) {
+        hipLaunchKernelGGL((k_col<S, C>), grid2(c->l, c->m, B2), B2, 0, c->s, L, P);
+    } else {
+        hipLaunchKernelGGL((k_colmc<S, C>), grid2(c->l, c->m, B2), B2, 0, c->s, L, P);
+        c->cur = 1 - c->cur;  // pointer swap of SRC/col_MC.F90:311-341
+    }
+    HIPCHK(c, hipGetLastError());
+    return BGK_OK;
+}
+
+bool foldable(const bgk_ctx *c) { return c->p.impl == BGK_IMPL_FUSED && c->p.bc != BGK_BC_STEP; }
+
+template <class S, class C, int BC, bool OBS>
+void launch_fused_t(bgk_ctx *c)
+{
+    Lat L = lat(c);
+    Phys<C> P = phys<C>(c);
+    hipLaunchKernelGGL((k_fused<S, C, BC, OBS>), grid2(c->l, c->m, B2), B2, 0, c->s, L, P);
+    c->cur = 1 - c->cur;
+}
+
+template <class S, class C, bool OBS>
+void launch_fused_o(bgk_ctx *c)
+{
+    switch (c->p.bc) {
+    case BGK_BC_PERIODIC: launch_fused_t<S, C, BK_PERIODIC, OBS>(c); break;
+    case BGK_BC_CHANNEL: launch_fused_t<S, C, BK_CHANNEL, OBS>(c); break;
+    case BGK_BC_INFLOW: launch_fused_t<S, C, BK_INFLOW, OBS>(c); break;
+    case BGK_BC_COUETTE: launch_fused_t<S, C, BK_COUETTE, OBS>(c); break;
+    default: launch_fused_t<S, C, BK_DRIVEN, OBS>(c); break;
+    }
+}
+
+template <class S, class C>
+int launch_step(bgk_ctx *c)
+{
+    if (foldable(c)) {
+        if (c->p.obstacle && c->nobs > 0)
+            launch_fused_o<S, C, true>(c);
+        else
+            launch_fused_o<S, C, false>(c);
+        HIPCHK(c, hipGetLastError());
+        return BGK_OK;
+    }
+    int r = launch_boundaries<S, C>(c);
+    if (r == BGK_OK) r = launch_propagation<S, C>(c);
+    if (r == BGK_OK) r = launch_collision<S, C>(c);
+    return r;
+}
+
+template <class S, class C>
+int run_t(bgk_ctx *c, int nsteps)
+{
+    HIPCHK(c, hipEventRecord(c->ev0, c->s));
+    for (int t = 0; t < nsteps; t++) {
+        int r = launch_step<S, C>(c);
+        if (r != BGK_OK) return r;
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->s));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = nsteps > 0 ? (double)ms / nsteps : 0.0;
+    return BGK_OK;
+}
+
+// sin and cos of one argument through the C library's sincos: the
+// reference's build (and gcc's) fuse the sin/cos pair of SRC/init.F90:109-110
+// into one sincos call, whose last bit can differ from separate sin and cos
+template <class C>
+void host_sincos(C arg, C *s, C *c);
+template <>
+void host_sincos<float>(float arg, float *s, float *c) { sincosf(arg, s, c); }
+template <>
+void host_sincos<double>(double arg, double *s, double *c) { sincos(arg, s, c); }
+
+template <class S, class C>
+int init_t(bgk_ctx *c, double amp)
+{
+    const bool tgv = c->p.bc == BGK_BC_PERIODIC;
+    const int nx = c->l + 2, ny = c->m + 2;
+    std::vector<C> tsx(nx), tcx(nx), tsy(ny), tcy(ny);
+    // SRC/init.F90:88-111: pi is a default-real literal in kind mykind,
+    // x = (real(i)-0.5d0)/real(l) evaluated in double, sin/cos in mykind
+    const C pi = (C)(float)3.141592653589793238462643383279;
+    for (int i = 0; i < nx; i++) {
+        C x = (C)(((double)(C)i - 0.5) / (double)(C)c->l);
+        host_sincos<C>((C)2 * pi * x, &tsx[i], &tcx[i]);
+    }
+    for (int j = 0; j < ny; j++) {
+        C y = (C)(((double)(C)j - 0.5) / (double)(C)c->m);
+        host_sincos<C>((C)2 * pi * y, &tsy[j], &tcy[j]);
+    }
+    C *d = nullptr;
+    HIPCHK(c, hipMalloc(&d, sizeof(C) * 2 * (nx + ny)));
+    HIPCHK(c, hipMemcpyAsync(d, tsx.data(), sizeof(C) * nx, hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(d + nx, tcx.data(), sizeof(C) * nx, hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(d + 2 * nx, tsy.data(), sizeof(C) * ny, hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(d + 2 * nx + ny, tcy.data(), sizeof(C) * ny, hipMemcpyHostToDevice, c->s));
+    Lat L = lat(c);
+    Phys<C> P = phys<C>(c);
+    hipLaunchKernelGGL((k_init<S, C>), grid2(nx, ny, B2), B2, 0, c->s, L, P, d, d + nx, d + 2 * nx,
+                       d + 2 * nx + ny, amp, tgv ? 1 : 0);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    HIPCHK(c, hipFree(d));
+    return BGK_OK;
+}
+
+template <class S, class C>
+int rows_t(bgk_ctx *c, bool final_)
+{
+    Lat L = lat(c);
+    Phys<C> P = phys<C>(c);
+    hipLaunchKernelGGL((k_rows<S, C>), dim3(c->m), dim3(ROWS_T), 0, c->s, L, P, c->d_rows);
+    if (final_) hipLaunchKernelGGL(k_rows_final, dim3(1), dim3(ROWS_T), 0, c->s, c->d_rows, c->m, c->d_red);
+    HIPCHK(c, hipGetLastError());
+    return BGK_OK;
+}
+
+template <class S, class C>
+int line_t(bgk_ctx *c, int mode, int c0, int c1, int n)
+{
+    Lat L = lat(c);
+    Phys<C> P = phys<C>(c);
+    hipLaunchKernelGGL((k_line<S, C>), dim3((n + 255) / 256), dim3(256), 0, c->s, L, P, mode, c0, c1, c->d_red);
+    HIPCHK(c, hipGetLastError());
+    return BGK_OK;
+}
+
+template <class S>
+int fill_t(bgk_ctx *c, S v)
+{
+    long long n = (long long)c->plane_elems * 17;
+    hipLaunchKernelGGL((k_fill<S>), dim3(2048), dim3(256), 0, c->s, c->mem, n, v);
+    HIPCHK(c, hipGetLastError());
+    return BGK_OK;
+}
+
+template <class S>
+int pack_t(bgk_ctx *c, int k, bool up)
+{
+    Lat L = lat(c);
+    void *plane = k == 8 ? c->plane[16] : c->plane[c->cur * 8 + k];
+    if (up)
+        hipLaunchKernelGGL((k_unpack<S>), grid2(c->l + 2, c->m + 2, B2), B2, 0, c->s, L, plane, (const S *)c->d_stage);
+    else
+        hipLaunchKernelGGL((k_pack<S>), grid2(c->l + 2, c->m + 2, B2), B2, 0, c->s, L, (const void *)plane,
+                           (S *)c->d_stage);
+    HIPCHK(c, hipGetLastError());
+    return BGK_OK;
+}
+
+// precision dispatch: storage kind S and compute kind C of each mode
+#define DISPATCH(c, FN, ...)                                                              \
+    [&]() -> int {                                                                        \
+        switch ((c)->p.prec) {                                                            \
+        case BGK_PREC_DOUBLE: return FN<double, double>((c), ##__VA_ARGS__);              \
+        case BGK_PREC_FP16: return FN<_Float16, float>((c), ##__VA_ARGS__);               \
+        case BGK_PREC_MIXED2: return FN<float, double>((c), ##__VA_ARGS__);               \
+        case BGK_PREC_BF16: return FN<bgk::bf16_t, float>((c), ##__VA_ARGS__);            \
+        default: return FN<float, float>((c), ##__VA_ARGS__);                             \
+        }                                                                                 \
+    }()
+
+int check_ctx(bgk_ctx *c)
+{
+    if (!c) return BGK_E_INVALID;
+    hipError_t e = hipSetDevice(c->dev);
+    if (e != hipSuccess) return fail(c, BGK_E_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+    return BGK_OK;
+}
+
+int fetch_red(bgk_ctx *c, int n)
+{
+    HIPCHK(c, hipMemcpyAsync(c->h_red, c->d_red, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return BGK_OK;
+}
+
+}  // namespace
+
+// =====================================================================
+extern "C" {
+
+int bgk_version(void) { return BGK_ABI_VERSION; }
+
+const char *bgk_last_error(const bgk_ctx_t *c) { return c ? c->err.c_str() : "null context"; }
+
+int bgk_create(const bgk_params_t *p, bgk_ctx_t **out)
+{
+    if (!p || !out) return BGK_E_INVALID;
+    *out = nullptr;
+    if (p->lx < 2 || p->ly < 2 || p->lx > (1 << 20) || p->ly > (1 << 20)) return BGK_E_INVALID;
+    if (p->bc < 0 || p->bc > 5 || p->prec < 0 || p->prec > 4 || (p->impl != 0 && p->impl != 1))
+        return BGK_E_INVALID;
+    if (p->bc == BGK_BC_STEP && p->lx < 20) return BGK_E_INVALID;  // stepx = l/20 must be >= 1
+    if (!(p->svisc > 0.0)) return BGK_E_INVALID;
+    bgk_ctx *c = new bgk_ctx();
+    c->p = *p;
+    c->l = p->lx;
+    c->m = p->ly;
+    c->dev = p->device;
+    c->scale = (p->scale > 0.0) ? p->scale : 1.0;
+    if (!is_fp16(p->prec) && !is_bf16(p->prec)) c->scale = 1.0;
+    {
+        int ex;
+        double fr = frexp(c->scale, &ex);
+        if (fr != 0.5) {
+            delete c;
+            return BGK_E_INVALID;  // power of two only: scaling must be exact
+        }
+    }
+    c->esize = esize_of(p->prec);
+    c->A = 128 / c->esize;
+    c->off = c->A - 1;
+    c->pitch = ((c->A + c->l + 1) + c->A - 1) / c->A * c->A;
+    size_t pe = (size_t)(c->m + 2) * (size_t)c->pitch;
+    const size_t align = 256 / c->esize;
+    c->plane_elems = (pe + align - 1) / align * align;
+    c->imin = c->l;
+    c->jmin = c->m;
+    c->imax = 0;
+    c->jmax = 0;
+    c->nobs = 0;
+    c->cur = 0;
+    c->last_ms = 0.0;
+    c->inited = false;
+    c->stage_bytes = 0;
+    c->d_stage = nullptr;
+    if (hipSetDevice(c->dev) != hipSuccess) {
+        delete c;
+        return BGK_E_HIP;
+    }
+    if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return BGK_E_HIP;
+    }
+    hipEventCreate(&c->ev0);
+    hipEventCreate(&c->ev1);
+    size_t bytes = c->plane_elems * (size_t)c->esize * 17;
+    size_t obs_bytes = ((size_t)c->l * c->m + 255) / 256 * 256;
+    size_t rows_bytes = sizeof(double) * ROW_VALS * (size_t)c->m;
+    size_t red_bytes = sizeof(double) * 3 * (size_t)(c->l > c->m ? c->l : c->m) + 64;
+    if (hipMalloc(&c->mem, bytes) != hipSuccess) {
+        hipStreamDestroy(c->s);
+        delete c;
+        return BGK_E_NOMEM;
+    }
+    if (hipMalloc(&c->d_obs, obs_bytes) != hipSuccess || hipMalloc(&c->d_rows, rows_bytes) != hipSuccess ||
+        hipMalloc(&c->d_red, red_bytes) != hipSuccess || hipHostMalloc(&c->h_red, red_bytes) != hipSuccess) {
+        hipFree(c->mem);
+        hipStreamDestroy(c->s);
+        delete c;
+        return BGK_E_NOMEM;
+    }
+    for (int q = 0; q < 17; q++) c->plane[q] = (char *)c->mem + (size_t)q * c->plane_elems * c->esize;
+    hipMemsetAsync(c->d_obs, 0, obs_bytes, c->s);
+    c->h_obs.assign((size_t)c->l * c->m, 0);
+    // alloca: huge(mykind) = 2147483647 converted to the storage kind
+    int r;
+    switch (p->prec) {
+    case BGK_PREC_DOUBLE: r = fill_t<double>(c, 2147483647.0); break;
+    case BGK_PREC_FP16: r = fill_t<_Float16>(c, (_Float16)2147483647.0f); break;
+    case BGK_PREC_BF16: {
+        bgk::bf16_t v;
+        v.bits = 0x4f00;  // 2^31
+        r = fill_t<bgk::bf16_t>(c, v);
+    } break;
+    default: r = fill_t<float>(c, (float)2147483647); break;
+    }
+    if (r != BGK_OK) {
+        bgk_destroy(c);
+        return r;
+    }
+    if (p->prec == BGK_PREC_DOUBLE || p->prec == BGK_PREC_MIXED2)
+        derive<double>(c);
+    else
+        derive<float>(c);
+    hipStreamSynchronize(c->s);
+    *out = c;
+    return BGK_OK;
+}
+
+int bgk_destroy(bgk_ctx_t *c)
+{
+    if (!c) return BGK_E_INVALID;
+    hipSetDevice(c->dev);
+    hipStreamSynchronize(c->s);
+    for (auto &g : c->graphs) hipGraphExecDestroy(g.second);
+    hipFree(c->mem);
+    hipFree(c->d_obs);
+    hipFree(c->d_rows);
+    hipFree(c->d_red);
+    hipHostFree(c->h_red);
+    if (c->d_stage) hipFree(c->d_stage);
+    hipEventDestroy(c->ev0);
+    hipEventDestroy(c->ev1);
+    hipStreamDestroy(c->s);
+    delete c;
+    return BGK_OK;
+}
+
+int bgk_constants(const bgk_ctx_t *c, double out[4])
+{
+    if (!c || !out) return BGK_E_INVALID;
+    out[0] = c->omega;
+    out[1] = c->fgrad;
+    out[2] = c->u00;
+    out[3] = c->force;
+    return BGK_OK;
+}
+
+static int set_mask(bgk_ctx *c, const int32_t *obs)
+{
+    int imin = c->l, jmin = c->m, imax = 0, jmax = 0, n = 0;
+    std::vector<uint8_t> h((size_t)c->l * c->m);
+    for (int j = 1; j <= c->m; j++)
+        for (int i = 1; i <= c->l; i++) {
+            size_t q = (size_t)(i - 1) + (size_t)(j - 1) * c->l;
+            h[q] = obs[q] == 1 ? 1 : 0;
+            if (h[q]) {
+                n++;
+                if (i < imin) imin = i;
+                if (j < jmin) jmin = j;
+                if (i > imax) imax = i;
+                if (j > jmax) jmax = j;
+            }
+        }
+    if (n > 0 && (imin < 2 || imax > c->l - 1 || jmin < 2 || jmax > c->m - 1))
+        return fail(c, BGK_E_INVALID, "obstacle must not touch the first/last interior row or column");
+    c->imin = imin;
+    c->imax = imax;
+    c->jmin = jmin;
+    c->jmax = jmax;
+    c->nobs = n;
+    c->h_obs.assign(obs, obs + (size_t)c->l * c->m);
+    HIPCHK(c, hipMemcpyAsync(c->d_obs, h.data(), h.size(), hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    for (auto &g : c->graphs) hipGraphExecDestroy(g.second);
+    c->graphs.clear();
+    return BGK_OK;
+}
+
+int bgk_build_obs(bgk_ctx_t *c)
+{
+    int r = check_ctx(c);
+    if (r) return r;
+    // SRC/build.obs.F90:170-205: obs = 1 where (icoord-i)^2+(jcoord-j)^2 <= R*R
+    // with the squared distance converted to mykind
+    const int l = c->l, m = c->m, ic = 2 * l / 5, jc = m / 2;
+    std::vector<int32_t> obs((size_t)l * m, 0);
+    const bool dbl = c->p.prec == BGK_PREC_DOUBLE || c->p.prec == BGK_PREC_MIXED2;
+    for (int j = 1; j <= m; j++)
+        for (int i = 1; i <= l; i++) {
+            long long d2i = (long long)(ic - i) * (ic - i) + (long long)(jc - j) * (jc - j);
+            bool in;
+            if (dbl) {
+                double R = (double)c->p.radius * 1.0;
+                in = (double)(int)d2i <= R * R;
+            } else {
+                float R = (float)c->p.radius * 1.0f;
+                in = (float)(int)d2i <= R * R;
+            }
+            obs[(size_t)(i - 1) + (size_t)(j - 1) * l] = in ? 1 : 0;
+        }
+    return set_mask(c, obs.data());
+}
+
+int bgk_set_obstacle(bgk_ctx_t *c, const int32_t *obs)
+{
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!obs) return fail(c, BGK_E_INVALID, "null mask");
+    return set_mask(c, obs);
+}
+
+int bgk_get_obstacle(bgk_ctx_t *c, int32_t *obs, int32_t bbox[4])
+{
+    if (!c) return BGK_E_INVALID;
+    if (obs) memcpy(obs, c->h_obs.data(), sizeof(int32_t) * c->h_obs.size());
+    if (bbox) {
+        bbox[0] = c->imin;
+        bbox[1] = c->imax;
+        bbox[2] = c->jmin;
+        bbox[3] = c->jmax;
+    }
+    return BGK_OK;
+}
+
+int bgk_init(bgk_ctx_t *c, int32_t init_v, double amplitude)
+{
+    int r = check_ctx(c);
+    if (r) return r;
+    if (init_v < 0 || init_v > 5) return fail(c, BGK_E_INVALID, "Initial condition out of range[0,5] %d", init_v);
+    r = DISPATCH(c, init_t, amplitude);
+    if (r == BGK_OK) c->inited = true;
+    return r;
+}
+
+int bgk_boundaries(bgk_ctx_t *c)
+{
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!c->inited) return fail(c, BGK_E_STATE, "bgk_init not called");
+    return DISPATCH(c, launch_boundaries);
+}
+
+int bgk_propagation(bgk_ctx_t *c)
+{
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!c->inited) return fail(c, BGK_E_STATE, "bgk_init not called");
+    return DISPATCH(c, launch_propagation);
+}
+
+int bgk_collision(bgk_ctx_t *c, int32_t itime)
+{
+    (void)itime;  // only the LES/debug builds of the reference use it
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!c->inited) return fail(c, BGK_E_STATE, "bgk_init not called");
+    return DISPATCH(c, launch_collision);
+}
+
+int bgk_run(bgk_ctx_t *c, int32_t nsteps)
+{
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!c->inited) return fail(c, BGK_E_STATE, "bgk_init not called");
+    if (nsteps < 0) return fail(c, BGK_E_INVALID, "negative step count");
+    return DISPATCH(c, run_t, nsteps);
+}
+
+int bgk_sync(bgk_ctx_t *c)
+{
+    int r = check_ctx(c);
+    if (r) return r;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return BGK_OK;
+}
+
+int bgk_diagno(bgk_ctx_t *c, double out[4])
+{
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!out) return BGK_E_INVALID;
+    r = DISPATCH(c, rows_t, true);
+    if (r == BGK_OK) r = fetch_red(c, 6);
+    if (r) return r;
+    const double *h = c->h_red;
+    out[0] = (h[1] / c->l) / c->m;
+    out[1] = (h[2] / c->l) / c->m;
+    out[2] = (h[0] / c->l) / c->m;
+    out[3] = (h[3] / c->l) / c->m;
+    return BGK_OK;
+}
+
+int bgk_check(bgk_ctx_t *c, double umax)
+{
+    int r = check_ctx(c);
+    if (r) return r;
+    r = DISPATCH(c, rows_t, true);
+    if (r == BGK_OK) r = fetch_red(c, 6);
+    if (r) return r;
+    if (c->h_red[5] > 0.0) return fail(c, BGK_E_DIVERGED, "non-finite populations");
+    if (c->h_red[4] > umax * umax) return fail(c, BGK_E_DIVERGED, "|u| = %g > %g", sqrt(c->h_red[4]), umax);
+    return BGK_OK;
+}
+
+int bgk_varm(bgk_ctx_t *c, double *u, double *v, double *rho)
+{
+    int r = check_ctx(c);
+    if (r) return r;
+    r = DISPATCH(c, rows_t, false);
+    if (r) return r;
+    std::vector<double> rows((size_t)ROW_VALS * c->m);
+    HIPCHK(c, hipMemcpyAsync(rows.data(), c->d_rows, sizeof(double) * rows.size(), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    for (int j = 0; j < c->m; j++) {
+        const double *q = &rows[(size_t)j * ROW_VALS];
+        if (rho) rho[j] = q[4] / c->l;
+        if (u) u[j] = q[5] / c->l;
+        if (v) v[j] = q[6] / c->l;
+    }
+    return BGK_OK;
+}
+
+static int line_common(bgk_ctx *c, int mode, int c0, int c1, int n, double *u, double *v, double *rho)
+{
+    int r = check_ctx(c);
+    if (r) return r;
+    r = DISPATCH(c, line_t, mode, c0, c1, n);
+    if (r == BGK_OK) r = fetch_red(c, 3 * n);
+    if (r) return r;
+    for (int t = 0; t < n; t++) {
+        if (u) u[t] = c->h_red[t];
+        if (v) v[t] = c->h_red[n + t];
+        if (rho) rho[t] = c->h_red[2 * n + t];
+    }
+    return BGK_OK;
+}
+
+int bgk_prof_i(bgk_ctx_t *c, int32_t jc, double *u, double *v, double *rho)
+{
+    if (!c || jc < 0 || jc > c->m + 1) return BGK_E_INVALID;
+    return line_common(c, 0, jc, 0, c->l, u, v, rho);
+}
+
+int bgk_prof_j(bgk_ctx_t *c, int32_t ic, double *u, double *v, double *rho)
+{
+    if (!c || ic < 0 || ic > c->l + 1) return BGK_E_INVALID;
+    return line_common(c, 1, ic, 0, c->m, u, v, rho);
+}
+
+int bgk_probe(bgk_ctx_t *c, int32_t i0, int32_t j0, double out[3])
+{
+    if (!c || !out || i0 < 0 || i0 > c->l + 1 || j0 < 0 || j0 > c->m + 1) return BGK_E_INVALID;
+    return line_common(c, 2, i0, j0, 1, &out[0], &out[1], &out[2]);
+}
+
+int bgk_storage_bytes(const bgk_ctx_t *c) { return c ? c->esize : BGK_E_INVALID; }
+
+static int ensure_stage(bgk_ctx *c)
+{
+    size_t need = (size_t)(c->l + 2) * (c->m + 2) * c->esize;
+    if (c->stage_bytes >= need) return BGK_OK;
+    if (c->d_stage) hipFree(c->d_stage);
+    c->d_stage = nullptr;
+    c->stage_bytes = 0;
+    HIPCHK(c, hipMalloc(&c->d_stage, need));
+    c->stage_bytes = need;
+    return BGK_OK;
+}
+
+int bgk_download(bgk_ctx_t *c, int32_t k, void *host)
+{
+    int r = check_ctx(c);
+    if (r) return r;
+    if (k < 0 || k > 8 || !host) return fail(c, BGK_E_INVALID, "bad population index %d", k);
+    if ((r = ensure_stage(c))) return r;
+    switch (c->esize) {
+    case 8: r = pack_t<double>(c, k, false); break;
+    case 2: r = pack_t<uint16_t>(c, k, false); break;
+    default: r = pack_t<float>(c, k, false); break;
+    }
+    if (r) return r;
+    size_t n = (size_t)(c->l + 2) * (c->m + 2) * c->esize;
+    HIPCHK(c, hipMemcpyAsync(host, c->d_stage, n, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return BGK_OK;
+}
+
+int bgk_upload(bgk_ctx_t *c, int32_t k, const void *host)
+{
+    int r = check_ctx(c);
+    if (r) return r;
+    if (k < 0 || k > 8 || !host) return fail(c, BGK_E_INVALID, "bad population index %d", k);
+    if ((r = ensure_stage(c))) return r;
+    size_t n = (size_t)(c->l + 2) * (c->m + 2) * c->esize;
+    HIPCHK(c, hipMemcpyAsync(c->d_stage, host, n, hipMemcpyHostToDevice, c->s));
+    switch (c->esize) {
+    case 8: r = pack_t<double>(c, k, true); break;
+    case 2: r = pack_t<uint16_t>(c, k, true); break;
+    default: r = pack_t<float>(c, k, true); break;
+    }
+    if (r) return r;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    c->inited = true;
+    return BGK_OK;
+}
+
+void *bgk_stream(bgk_ctx_t *c) { return c ? (void *)c->s : nullptr; }
+
+int bgk_last_run_ms(const bgk_ctx_t *c, double *ms)
+{
+    if (!c || !ms) return BGK_E_INVALID;
+    *ms = c->last_ms;
+    return BGK_OK;
+}
+
+}  // extern "C"

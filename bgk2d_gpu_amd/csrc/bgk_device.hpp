@@ -1,0 +1,181 @@
+// bgk_device.hpp -- device-side building blocks of the gfx950 BGK2D path.
+//
+// Layout in HBM (one allocation per context):
+//   17 planes: pops 01,03,05,08,10,12,14,17 of set 0, the same of set 1,
+//   and the rest population 19 (a single plane updated in place, as
+//   SRC/col_MC.F90:293 does).  A plane holds (m+2) rows of `pitch`
+//   elements; node (i,j), 0 <= i <= l+1, 0 <= j <= m+1, sits at element
+//   off + j*pitch + i with off = (128 B / element size) - 1, so that the
+//   first interior node i = 1 of every row starts a 128-byte line and a
+//   64-lane wavefront reading 64 consecutive nodes is fully coalesced.
+//
+// All arithmetic follows the reference's operation order and is compiled
+// with -ffp-contract=off, so every value is bit-identical to the reference
+// CPU build on the same inputs (x86-64 flang makes no FMA either).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bgk {
+
+// population index k = 0..8  <->  01 03 05 08 10 12 14 17 19
+__host__ __device__ constexpr int cx_of(int k) { return k < 3 ? 1 : (k == 3 ? 0 : (k < 7 ? -1 : 0)); }
+__host__ __device__ constexpr int cy_of(int k)
+{
+    return k == 0 ? -1 : k == 1 ? 1 : k == 2 ? 0 : k == 3 ? 1 : k == 4 ? -1 : k == 5 ? 1 : k == 6 ? 0 : k == 7 ? -1 : 0;
+}
+__host__ __device__ constexpr int opp_of(int k)
+{
+    return k == 0 ? 5 : k == 1 ? 4 : k == 2 ? 6 : k == 3 ? 7 : k == 4 ? 1 : k == 5 ? 0 : k == 6 ? 2 : k == 7 ? 3 : 8;
+}
+
+// ------------------------------------------------------------ storage
+// A codec maps one stored element to the compute kind and back.  FP16 and
+// BF16 hold scale*(f - w) with a power-of-two scale; scaling by a power
+// of two is exact, so scale = 1 reproduces the reference's MIXED1 storage.
+struct bf16_t {
+    uint16_t bits;
+};
+
+template <class S, class C>
+struct Codec;
+
+template <>
+struct Codec<float, float> {
+    using raw = float;
+    __device__ __forceinline__ static float dec(float v, float) { return v; }
+    __device__ __forceinline__ static float enc(float v, float) { return v; }
+};
+template <>
+struct Codec<double, double> {
+    using raw = double;
+    __device__ __forceinline__ static double dec(double v, double) { return v; }
+    __device__ __forceinline__ static double enc(double v, double) { return v; }
+};
+template <>
+struct Codec<float, double> {  // MIXED2
+    using raw = float;
+    __device__ __forceinline__ static double dec(float v, double) { return (double)v; }
+    __device__ __forceinline__ static float enc(double v, double) { return (float)v; }
+};
+template <>
+struct Codec<_Float16, float> {
+    using raw = _Float16;
+    // dec/enc take the scale factor that applies: inv_scale / scale
+    __device__ __forceinline__ static float dec(_Float16 v, float inv) { return (float)v * inv; }
+    __device__ __forceinline__ static _Float16 enc(float v, float sc) { return (_Float16)(v * sc); }
+};
+template <>
+struct Codec<bf16_t, float> {
+    using raw = bf16_t;
+    __device__ __forceinline__ static float dec(bf16_t v, float inv)
+    {
+        return __uint_as_float((uint32_t)v.bits << 16) * inv;
+    }
+    __device__ __forceinline__ static bf16_t enc(float v, float sc)
+    {
+        uint32_t x = __float_as_uint(v * sc);
+        bf16_t r;
+        if ((x & 0x7fffffffu) > 0x7f800000u) {
+            r.bits = (uint16_t)((x >> 16) | 0x40u);
+        } else {
+            x += 0x7fffu + ((x >> 16) & 1u);
+            r.bits = (uint16_t)(x >> 16);
+        }
+        return r;
+    }
+};
+
+// ---------------------------------------------------------- descriptors
+struct Lat {
+    void *a[8];  // current set, pops 01..17
+    void *b[8];  // other set (FUSED target / ORIGINAL streamed copy)
+    void *r;     // pop 19
+    const uint8_t *obs;  // l*m solid mask (x fastest) or null
+    int l, m;
+    int pitch, off;
+    int imin, imax, jmin, jmax;  // obstacle bounding box (1-based)
+    int stepx, stepy;
+    __device__ __forceinline__ long long at(int i, int j) const { return (long long)off + (long long)j * pitch + i; }
+};
+
+template <class C>
+struct Phys {
+    C omega, fgrad, force;  // hencol / bcond_driven constants
+    C sc, inv;              // storage scale and its inverse (FP16/BF16)
+    C inflow[3];            // stored-and-decoded inflow equilibria 01,03,05
+    int channel_force;      // 1: CHANNEL body force in collision
+};
+
+// ---------------------------------------------------------- arithmetic
+// Reference constants (SRC/storage.F90:44-73,130-138): evaluated in double
+// precision, then converted to the compute kind.
+template <class C>
+struct K {
+    static constexpr C rf = (C)3.0;
+    static constexpr C qf = (C)1.5;
+    static constexpr C p0 = (C)(4.0 / 9.0);
+    static constexpr C p1 = (C)(1.0 / 9.0);
+    static constexpr C p2 = (C)(1.0 / (3.0 * 3.0 * (1.0 + 3.0)));
+};
+
+// BGK relaxation of one node in the shifted form, operation for operation
+// as SRC/col_MC.F90:125-293 (and SRC/col.F90:104-237).
+template <class C>
+__device__ __forceinline__ void relax(const Phys<C> &P, const C x[9], C o[9])
+{
+    const C cte1 = (C)1.0, cte0 = (C)1.0 - cte1;
+    const C rf = K<C>::rf, qf = K<C>::qf, p0 = K<C>::p0, p1 = K<C>::p1, p2 = K<C>::p2;
+    const C x01 = x[0], x03 = x[1], x05 = x[2], x08 = x[3], x10 = x[4];
+    const C x12 = x[5], x14 = x[6], x17 = x[7], x19 = x[8];
+    C rho = (((x01 + x03 + x10 + x12) + (x05 + x08 + x14 + x17)) + x19) + cte1;
+    C rhoinv = (C)1.0 / rho;
+    C vx = ((x03 - x12) + (x01 - x10) + (x05 - x14)) * rhoinv;
+    C vy = ((x03 - x01) + (x12 - x10) + (x08 - x17)) * rhoinv;
+    C vx2 = vx * vx, vy2 = vy * vy;
+    C vsq = vx2 + vy2;
+    C vxpy = vx + vy, vxmy = vx - vy;
+    C qxpy = cte0 + qf * ((C)3.0 * vxpy * vxpy - vsq);
+    C qxmy = cte0 + qf * ((C)3.0 * vxmy * vxmy - vsq);
+    C qx = cte0 + qf * ((C)3.0 * vx2 - vsq);
+    C qy = cte0 + qf * ((C)3.0 * vy2 - vsq);
+    C q0 = cte0 + qf * (-vsq);
+    vx = rf * vx;
+    vy = rf * vy;
+    vxpy = rf * vxpy;
+    vxmy = rf * vxmy;
+    C rp0 = rho * p0, rp1 = rho * p1, rp2 = rho * p2;
+    C e01 = rp2 * (+vxmy + qxmy) + cte1 * (rp2 - p2);
+    C e03 = rp2 * (+vxpy + qxpy) + cte1 * (rp2 - p2);
+    C e05 = rp1 * (+vx + qx) + cte1 * (rp1 - p1);
+    C e08 = rp1 * (+vy + qy) + cte1 * (rp1 - p1);
+    C e10 = rp2 * (-vxpy + qxpy) + cte1 * (rp2 - p2);
+    C e12 = rp2 * (-vxmy + qxmy) + cte1 * (rp2 - p2);
+    C e14 = rp1 * (-vx + qx) + cte1 * (rp1 - p1);
+    C e17 = rp1 * (-vy + qy) + cte1 * (rp1 - p1);
+    C e19 = rp0 * (+q0) + cte1 * (rp0 - p0);
+    C forcex = (C)0.0, forcey = (C)0.0;
+    if (P.channel_force) forcex = P.fgrad * rho;
+    const C om = P.omega;
+    o[0] = x01 - om * (x01 - e01) + (+forcex - forcey);
+    o[1] = x03 - om * (x03 - e03) + (+forcex + forcey);
+    o[2] = x05 - om * (x05 - e05) + (+forcex);
+    o[3] = x08 - om * (x08 - e08) + (+forcey);
+    o[4] = x10 - om * (x10 - e10) + (-forcex - forcey);
+    o[5] = x12 - om * (x12 - e12) + (-forcex + forcey);
+    o[6] = x14 - om * (x14 - e14) + (-forcex);
+    o[7] = x17 - om * (x17 - e17) + (-forcey);
+    o[8] = x19 - om * (x19 - e19);
+}
+
+// Shifted equilibrium at rho = 1 with the quadratic part in double
+// precision (the 3.d0 literal of SRC/init.F90:114-122 and
+// SRC/bcond.inflow.F90:104-106).
+template <class C>
+__host__ __device__ inline C eq_d(C cu, C cvsq)
+{
+    double q = 3.0 * (double)cu * (double)cu - (double)cvsq;
+    return (C)((double)(K<C>::rf * cu) + (double)K<C>::qf * q);
+}
+
+}  // namespace bgk

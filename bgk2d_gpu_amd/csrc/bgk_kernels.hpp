@@ -1,0 +1,551 @@
+// bgk_kernels.hpp -- gfx950 kernels of the BGK2D time step.
+//
+// Kernel inventory (reference loop each one replaces):
+//   k_fill        alloca's huge() fill            SRC/alloca.F90:133-151
+//   k_init        init                            SRC/init.F90:83-137
+//   k_bc          the 1-D halo passes of every bcond_* variant
+//   k_bc_obs      bcond_obs                       SRC/bcond.obs.F90:50-59
+//   k_movef       movef (ORIGINAL stream)         SRC/movef.F90:46-53
+//   k_col         col (ORIGINAL collide)          SRC/col.F90:90-237
+//   k_colmc       col_MC (FUSED pull + collide)   SRC/col_MC.F90:114-293
+//   k_fused       col_MC with the boundary folded into the pull (no halo
+//                 pass): the hot kernel of bgk_run
+//   k_rows/k_rows_final  diagno + varm + divergence check, FP64
+//   k_line        prof_i / prof_j / probe
+//   k_pack/k_unpack      population <-> dense host layout
+#pragma once
+#include "bgk_device.hpp"
+
+namespace bgk {
+
+template <class S, class C>
+using CodecT = Codec<S, C>;
+
+template <class S>
+__device__ __forceinline__ S &el(void *p, long long idx) { return ((S *)p)[idx]; }
+template <class S>
+__device__ __forceinline__ S elc(const void *p, long long idx) { return ((const S *)p)[idx]; }
+
+// decoded read / encoded write of population k (0..7 in a set, 8 = rest)
+template <class S, class C>
+__device__ __forceinline__ C get(const Lat &L, const Phys<C> &P, void *const *set, int k, int i, int j)
+{
+    const void *p = (k == 8) ? L.r : set[k];
+    return CodecT<S, C>::dec(elc<S>(p, L.at(i, j)), P.inv);
+}
+template <class S, class C>
+__device__ __forceinline__ void put(const Lat &L, const Phys<C> &P, void *const *set, int k, int i, int j, C v)
+{
+    void *p = (k == 8) ? L.r : set[k];
+    el<S>(p, L.at(i, j)) = CodecT<S, C>::enc(v, P.sc);
+}
+// raw copy of a stored element (the reference's `a(x) = a(y)` assignments)
+template <class S>
+__device__ __forceinline__ void cpy(const Lat &L, void *dst, int di, int dj, const void *src, int si, int sj)
+{
+    el<S>(dst, L.at(di, dj)) = elc<S>(src, L.at(si, sj));
+}
+// storage round trip of a compute-kind value (a value stored to a halo by
+// a boundary and read back by the pull)
+template <class S, class C>
+__device__ __forceinline__ C rt(const Phys<C> &P, C v)
+{
+    return CodecT<S, C>::dec(CodecT<S, C>::enc(v, P.sc), P.inv);
+}
+
+// ------------------------------------------------------------ fill/init
+template <class S>
+__global__ void k_fill(void *base, long long n, S v)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) ((S *)base)[t] = v;
+}
+
+// init (SRC/init.F90:83-137): tx = {sin, cos}(2 pi x_i) per column and
+// ty = {sin, cos}(2 pi y_j) per row, evaluated on the host with the C
+// library's sin/cos in the compute kind as the reference's build does.
+template <class S, class C>
+__global__ void k_init(Lat L, Phys<C> P, const C *tsx, const C *tcx, const C *tsy, const C *tcy, double amp,
+                       int tgv)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > L.l + 1 || j > L.m + 1) return;
+    C xj = (C)0.0, yj = (C)0.0;
+    if (tgv) {
+        xj = (C)(amp * (double)tsx[i] * (double)tcy[j]);
+        yj = (C)(-(amp * (double)tcx[i] * (double)tsy[j]));
+    }
+    const C rf = K<C>::rf, qf = K<C>::qf, p0 = K<C>::p0, p1 = K<C>::p1, p2 = K<C>::p2;
+    const C crho = (C)1.0, cte1 = (C)0.0;
+    C cvsq = xj * xj + yj * yj;
+    C c01 = eq_d<C>(xj - yj, cvsq);
+    C c03 = eq_d<C>(xj + yj, cvsq);
+    C c05 = eq_d<C>(xj, cvsq);
+    C c08 = eq_d<C>(yj, cvsq);
+    C c10 = (C)((double)(rf * (-xj - yj)) + (double)qf * (3.0 * (double)(xj + yj) * (double)(xj + yj) - (double)cvsq));
+    C c12 = (C)((double)(rf * (-xj + yj)) + (double)qf * (3.0 * (double)(xj - yj) * (double)(xj - yj) - (double)cvsq));
+    C c14 = (C)((double)(rf * (-xj)) + (double)qf * (3.0 * (double)xj * (double)xj - (double)cvsq));
+    C c17 = (C)((double)(rf * (-yj)) + (double)qf * (3.0 * (double)yj * (double)yj - (double)cvsq));
+    C c19 = (C)((double)rf * 0.0 + (double)qf * (3.0 * 0.0 * 0.0 - (double)cvsq));
+    put<S, C>(L, P, L.a, 0, i, j, crho * p2 * (cte1 + c01));
+    put<S, C>(L, P, L.a, 1, i, j, crho * p2 * (cte1 + c03));
+    put<S, C>(L, P, L.a, 2, i, j, crho * p1 * (cte1 + c05));
+    put<S, C>(L, P, L.a, 3, i, j, crho * p1 * (cte1 + c08));
+    put<S, C>(L, P, L.a, 4, i, j, crho * p2 * (cte1 + c10));
+    put<S, C>(L, P, L.a, 5, i, j, crho * p2 * (cte1 + c12));
+    put<S, C>(L, P, L.a, 6, i, j, crho * p1 * (cte1 + c14));
+    put<S, C>(L, P, L.a, 7, i, j, crho * p1 * (cte1 + c17));
+    put<S, C>(L, P, L.a, 8, i, j, crho * p0 * (cte1 + c19));
+}
+
+// ------------------------------------------------------------ boundaries
+// One 1-D halo pass; the passes of a variant are launched in the order the
+// reference's loops run, so corner priorities are the reference's.
+enum BcPass {
+    BP_DRV_X = 0,    // bcond_driven x-walls, j = 1..m  SRC/bcond.driven.F90:96-109
+    BP_WALL_Y = 1,   // bottom wall + top wall (lid: +-u00/6), i = t0..t0+n-1
+    BP_XPER = 2,     // x periodic copy, j = t0..t0+n-1  SRC/bcond.periodic.F90:60-68
+    BP_YPER = 3,     // y periodic copy, i = 0..l+1      SRC/bcond.periodic.F90:99-107
+    BP_INFLOW_X = 4, // outflow + uniform inflow, j = 0..m+1  SRC/bcond.inflow.F90:80-113
+    BP_STEP_OUT = 5, // outflow only, j = 0..m+1        SRC/bcond.step.F90:66-102
+    BP_STEP_IN = 6,  // parabolic inflow, j = stepy..m+1 SRC/bcond.step.F90:116-151
+    BP_STEP_TOP = 7, // step top face, i = 1..stepx     SRC/bcond.step.F90:205-223
+    BP_STEP_FACE = 8 // step vertical face, j = 1..stepy SRC/bcond.step.F90:237-255
+};
+
+template <class S, class C>
+__device__ void outflow_col(const Lat &L, const Phys<C> &P, int j)
+{
+    const int l = L.l, l1 = l + 1;
+    const C rf = K<C>::rf, qf = K<C>::qf, p1 = K<C>::p1, p2 = K<C>::p2;
+    const C rhoinv = (C)1.0, crho = (C)1.0, cte1 = (C)0.0;
+    C x01 = get<S, C>(L, P, L.a, 0, l, j), x03 = get<S, C>(L, P, L.a, 1, l, j);
+    C x05 = get<S, C>(L, P, L.a, 2, l, j), x08 = get<S, C>(L, P, L.a, 3, l, j);
+    C x10 = get<S, C>(L, P, L.a, 4, l, j), x12 = get<S, C>(L, P, L.a, 5, l, j);
+    C x14 = get<S, C>(L, P, L.a, 6, l, j), x17 = get<S, C>(L, P, L.a, 7, l, j);
+    C xj = ((x03 - x12) + (x01 - x10) + (x05 - x14)) * rhoinv;
+    C yj = ((x03 - x01) + (x12 - x10) + (x08 - x17)) * rhoinv;
+    C cvsq = xj * xj + yj * yj;
+    C cx10 = rf * (-xj - yj) + qf * ((C)3.0 * (xj + yj) * (xj + yj) - cvsq);
+    C cx12 = rf * (-xj + yj) + qf * ((C)3.0 * (xj - yj) * (xj - yj) - cvsq);
+    C cx14 = rf * (-xj) + qf * ((C)3.0 * (xj) * (xj)-cvsq);
+    put<S, C>(L, P, L.a, 4, l1, j, crho * p2 * (cte1 + cx10));
+    put<S, C>(L, P, L.a, 5, l1, j, crho * p2 * (cte1 + cx12));
+    put<S, C>(L, P, L.a, 6, l1, j, crho * p1 * (cte1 + cx14));
+}
+
+template <class S, class C>
+__device__ void inflow_col(const Lat &L, const Phys<C> &P, int j, C xj)
+{
+    const C p1 = K<C>::p1, p2 = K<C>::p2;
+    const C crho = (C)1.0, cte1 = (C)0.0, yj = (C)0.0;
+    C cvsq = xj * xj + yj * yj;
+    C c01 = eq_d<C>(xj - yj, cvsq), c03 = eq_d<C>(xj + yj, cvsq), c05 = eq_d<C>(xj, cvsq);
+    put<S, C>(L, P, L.a, 0, 0, j, crho * p2 * (cte1 + c01));
+    put<S, C>(L, P, L.a, 1, 0, j, crho * p2 * (cte1 + c03));
+    put<S, C>(L, P, L.a, 2, 0, j, crho * p1 * (cte1 + c05));
+}
+
+template <class S, class C>
+__global__ void k_bc(Lat L, Phys<C> P, int pass, int t0, int n, int lid)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    t += t0;
+    const int l = L.l, m = L.m, l1 = l + 1, m1 = m + 1;
+    void *const *a = L.a;
+    switch (pass) {
+    case BP_DRV_X: {
+        int j = t;
+        cpy<S>(L, a[5], l1, j - 1, a[0], l, j);  // a12(l1,j-1) = a01(l,j)
+        cpy<S>(L, a[4], l1, j + 1, a[1], l, j);  // a10(l1,j+1) = a03(l,j)
+        cpy<S>(L, a[6], l1, j, a[2], l, j);      // a14(l1,j)   = a05(l,j)
+        cpy<S>(L, a[1], 0, j - 1, a[4], 1, j);   // a03(0,j-1)  = a10(1,j)
+        cpy<S>(L, a[0], 0, j + 1, a[5], 1, j);   // a01(0,j+1)  = a12(1,j)
+        cpy<S>(L, a[2], 0, j, a[6], 1, j);       // a05(0,j)    = a14(1,j)
+    } break;
+    case BP_WALL_Y: {
+        int i = t;
+        cpy<S>(L, a[3], i, 0, a[7], i, 1);                       // a08(i,0)   = a17(i,1)
+        if (i + 1 <= l1) cpy<S>(L, a[5], i + 1, 0, a[0], i, 1);  // a12(i+1,0) = a01(i,1)
+        if (i - 1 >= 0) cpy<S>(L, a[1], i - 1, 0, a[4], i, 1);   // a03(i-1,0) = a10(i,1)
+        if (!lid) {  // channel / step walls: plain copies
+            if (i + 1 <= l1) cpy<S>(L, a[4], i + 1, m1, a[1], i, m);  // a10(i+1,m1) = a03(i,m)
+            if (i - 1 >= 0) cpy<S>(L, a[0], i - 1, m1, a[5], i, m);   // a01(i-1,m1) = a12(i,m)
+        } else {      // driven / couette lid: a03(i,m) - force, a12(i,m) + force
+            if (i + 1 <= l1) put<S, C>(L, P, a, 4, i + 1, m1, get<S, C>(L, P, a, 1, i, m) - P.force);
+            if (i - 1 >= 0) put<S, C>(L, P, a, 0, i - 1, m1, get<S, C>(L, P, a, 5, i, m) + P.force);
+        }
+        cpy<S>(L, a[7], i, m1, a[3], i, m);  // a17(i,m1) = a08(i,m)
+    } break;
+    case BP_XPER: {
+        int j = t;
+        cpy<S>(L, a[0], 0, j, a[0], l, j);
+        cpy<S>(L, a[1], 0, j, a[1], l, j);
+        cpy<S>(L, a[2], 0, j, a[2], l, j);
+        cpy<S>(L, a[4], l1, j, a[4], 1, j);
+        cpy<S>(L, a[5], l1, j, a[5], 1, j);
+        cpy<S>(L, a[6], l1, j, a[6], 1, j);
+    } break;
+    case BP_YPER: {
+        int i = t;
+        cpy<S>(L, a[7], i, m1, a[7], i, 1);
+        cpy<S>(L, a[0], i, m1, a[0], i, 1);
+        cpy<S>(L, a[4], i, m1, a[4], i, 1);
+        cpy<S>(L, a[1], i, 0, a[1], i, m);
+        cpy<S>(L, a[3], i, 0, a[3], i, m);
+        cpy<S>(L, a[5], i, 0, a[5], i, m);
+    } break;
+    case BP_INFLOW_X: {
+        int j = t;
+        outflow_col<S, C>(L, P, j);
+        inflow_col<S, C>(L, P, j, (C)0.1f);
+    } break;
+    case BP_STEP_OUT: outflow_col<S, C>(L, P, t); break;
+    case BP_STEP_IN: {
+        int j = t;
+        const int stepy = L.stepy;
+        C xj = (C)0.1f * (C)(float)(j - stepy) * (C)(float)(m - j) / (C)(float)((m - stepy) * (m - stepy) / 4);
+        inflow_col<S, C>(L, P, j, xj);
+    } break;
+    case BP_STEP_TOP: {
+        int i = t;
+        const int sy = L.stepy;
+        cpy<S>(L, a[3], i, sy - 1, a[7], i, sy);
+        cpy<S>(L, a[5], i + 1, sy - 1, a[0], i, sy);
+        cpy<S>(L, a[1], i - 1, sy - 1, a[4], i, sy);
+    } break;
+    case BP_STEP_FACE: {
+        int j = t;
+        const int sx = L.stepx;
+        cpy<S>(L, a[1], sx - 1, j - 1, a[4], sx, j);
+        cpy<S>(L, a[0], sx - 1, j + 1, a[5], sx, j);
+        cpy<S>(L, a[2], sx - 1, j, a[6], sx, j);
+    } break;
+    default: break;
+    }
+}
+
+// bcond_obs without the serial loop: for a solid node s and population k
+// the reference sets a_k(s) = a_opp(k)(s + c_k) in j-outer/i-inner order.
+// When the neighbour n = s + c_k comes earlier in that order (c_k with
+// cy = -1, or cy = 0 and cx = -1: pops 01,10,14,17) and is itself solid,
+// it already copied a_k(s) into a_opp(k)(n), so a_k(s) keeps its value;
+// otherwise a_k(s) takes n's value, which no thread writes.  Threads that
+// would store an unchanged value skip the store, which makes every read
+// race-free and the result equal to the serial loop.
+__device__ __forceinline__ bool before_pop(int k) { return k == 0 || k == 4 || k == 6 || k == 7; }
+
+__device__ __forceinline__ bool solid(const Lat &L, int i, int j)
+{
+    if (i < L.imin || i > L.imax || j < L.jmin || j > L.jmax) return false;
+    return L.obs[(long long)(i - 1) + (long long)(j - 1) * L.l] != 0;
+}
+
+template <class S>
+__global__ void k_bc_obs(Lat L)
+{
+    int i = L.imin + blockIdx.x * blockDim.x + threadIdx.x;
+    int j = L.jmin + blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > L.imax || j > L.jmax || !solid(L, i, j)) return;
+    S v[8];
+    bool w[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        int ni = i + cx_of(k), nj = j + cy_of(k);
+        w[k] = !(before_pop(k) && solid(L, ni, nj));
+        if (w[k]) v[k] = elc<S>(L.a[opp_of(k)], L.at(ni, nj));
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if (w[k]) el<S>(L.a[k], L.at(i, j)) = v[k];
+}
+
+// ------------------------------------------------------- stream/collide
+template <class S>
+__global__ void k_movef(Lat L)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > L.l || j > L.m) return;
+#pragma unroll
+    for (int k = 0; k < 8; k++) cpy<S>(L, L.b[k], i, j, L.a[k], i - cx_of(k), j - cy_of(k));
+}
+
+template <class S, class C>
+__global__ void k_col(Lat L, Phys<C> P)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > L.l || j > L.m) return;
+    C x[9], o[9];
+#pragma unroll
+    for (int k = 0; k < 8; k++) x[k] = get<S, C>(L, P, L.b, k, i, j);
+    x[8] = get<S, C>(L, P, L.a, 8, i, j);
+    relax<C>(P, x, o);
+#pragma unroll
+    for (int k = 0; k < 9; k++) put<S, C>(L, P, L.a, k, i, j, o[k]);
+}
+
+template <class S, class C>
+__global__ void k_colmc(Lat L, Phys<C> P)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > L.l || j > L.m) return;
+    C x[9], o[9];
+#pragma unroll
+    for (int k = 0; k < 8; k++) x[k] = get<S, C>(L, P, L.a, k, i - cx_of(k), j - cy_of(k));
+    x[8] = get<S, C>(L, P, L.a, 8, i, j);
+    relax<C>(P, x, o);
+#pragma unroll
+    for (int k = 0; k < 8; k++) put<S, C>(L, P, L.b, k, i, j, o[k]);
+    put<S, C>(L, P, L.a, 8, i, j, o[8]);
+}
+
+// ------------------------------------------------ fused, boundary folded
+// Every boundary variant but STEP writes each halo value that the pull
+// reads as a function of post-collision interior values of the same step
+// (own reversed population +- lid term, periodic wrap, or the equilibrium
+// of a column-l node); the pull evaluates that function directly, so one
+// kernel per step replaces bcond_* + bcond_obs + col_MC and no halo is
+// ever written.  Rules, in the reference's corner priority:
+//   DRIVEN  : source row m+1 -> own opposite +/- u00/6 (lid wins corners);
+//             any other halo -> own opposite.
+//   PERIODIC: wrap both coordinates.
+//   CHANNEL : source row out -> own opposite; else wrap x.
+//   COUETTE : source row m+1 -> lid; row 0 -> own opposite; else wrap x.
+//   INFLOW  : wrap y first; column 0 -> uniform inflow equilibrium,
+//             column l+1 -> equilibrium of the momentum of (l, row).
+enum BcKind { BK_DRIVEN = 0, BK_PERIODIC = 1, BK_CHANNEL = 2, BK_INFLOW = 3, BK_COUETTE = 4 };
+
+template <class S, class C, int BC>
+__device__ C folded(const Lat &L, const Phys<C> &P, int k, int i, int j, int si, int sj)
+{
+    const int l = L.l, m = L.m;
+    if (BC == BK_PERIODIC) {
+        si = si < 1 ? l : (si > l ? 1 : si);
+        sj = sj < 1 ? m : (sj > m ? 1 : sj);
+        return get<S, C>(L, P, L.a, k, si, sj);
+    }
+    if (BC == BK_INFLOW) {
+        sj = sj < 1 ? m : (sj > m ? 1 : sj);
+        if (si < 1) return P.inflow[k];  // only pops 01,03,05 come from x = 0
+        if (si > l) {
+            // outflow equilibrium from the momentum of (l, sj)
+            const C rf = K<C>::rf, qf = K<C>::qf, p1 = K<C>::p1, p2 = K<C>::p2;
+            const C rhoinv = (C)1.0, crho = (C)1.0, cte1 = (C)0.0;
+            C x01 = get<S, C>(L, P, L.a, 0, l, sj), x03 = get<S, C>(L, P, L.a, 1, l, sj);
+            C x05 = get<S, C>(L, P, L.a, 2, l, sj), x08 = get<S, C>(L, P, L.a, 3, l, sj);
+            C x10 = get<S, C>(L, P, L.a, 4, l, sj), x12 = get<S, C>(L, P, L.a, 5, l, sj);
+            C x14 = get<S, C>(L, P, L.a, 6, l, sj), x17 = get<S, C>(L, P, L.a, 7, l, sj);
+            C xj = ((x03 - x12) + (x01 - x10) + (x05 - x14)) * rhoinv;
+            C yj = ((x03 - x01) + (x12 - x10) + (x08 - x17)) * rhoinv;
+            C cvsq = xj * xj + yj * yj;
+            C v;
+            if (k == 4)
+                v = crho * p2 * (cte1 + (rf * (-xj - yj) + qf * ((C)3.0 * (xj + yj) * (xj + yj) - cvsq)));
+            else if (k == 5)
+                v = crho * p2 * (cte1 + (rf * (-xj + yj) + qf * ((C)3.0 * (xj - yj) * (xj - yj) - cvsq)));
+            else
+                v = crho * p1 * (cte1 + (rf * (-xj) + qf * ((C)3.0 * (xj) * (xj)-cvsq)));
+            return rt<S, C>(P, v);
+        }
+        return get<S, C>(L, P, L.a, k, si, sj);
+    }
+    // wall bounded in y: DRIVEN, CHANNEL, COUETTE
+    const int o = opp_of(k);
+    if (sj > m && (BC == BK_DRIVEN || BC == BK_COUETTE)) {
+        C own = get<S, C>(L, P, L.a, o, i, j);
+        const int c = cx_of(k);
+        if (c == 0) return own;
+        return rt<S, C>(P, c > 0 ? own + P.force : own - P.force);
+    }
+    if (sj < 1 || sj > m) return get<S, C>(L, P, L.a, o, i, j);
+    if (BC == BK_DRIVEN) return get<S, C>(L, P, L.a, o, i, j);
+    si = si < 1 ? l : (si > l ? 1 : si);  // CHANNEL / COUETTE: x periodic
+    return get<S, C>(L, P, L.a, k, si, sj);
+}
+
+template <class S, class C, int BC, bool OBS>
+__global__ __launch_bounds__(256) void k_fused(Lat L, Phys<C> P)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > L.l || j > L.m) return;
+    C x[9], o[9];
+    // plain pulls; the halo ring is allocated, so edge nodes read it too
+    // and then replace those values below
+#pragma unroll
+    for (int k = 0; k < 8; k++) x[k] = get<S, C>(L, P, L.a, k, i - cx_of(k), j - cy_of(k));
+    x[8] = get<S, C>(L, P, L.a, 8, i, j);
+    if (i == 1 || i == L.l || j == 1 || j == L.m) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            int si = i - cx_of(k), sj = j - cy_of(k);
+            if (si < 1 || si > L.l || sj < 1 || sj > L.m) x[k] = folded<S, C, BC>(L, P, k, i, j, si, sj);
+        }
+    }
+    if (OBS && i >= L.imin - 1 && i <= L.imax + 1 && j >= L.jmin - 1 && j <= L.jmax + 1) {
+        const bool me = solid(L, i, j);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            int si = i - cx_of(k), sj = j - cy_of(k);
+            // the pulled value was set by bcond_obs: keep it only when this
+            // node is solid and precedes the source in the serial order
+            if (solid(L, si, sj) && !(me && before_pop(k))) x[k] = get<S, C>(L, P, L.a, opp_of(k), i, j);
+        }
+    }
+    relax<C>(P, x, o);
+#pragma unroll
+    for (int k = 0; k < 8; k++) put<S, C>(L, P, L.b, k, i, j, o[k]);
+    put<S, C>(L, P, L.a, 8, i, j, o[8]);
+}
+
+// ------------------------------------------------------------ diagnostics
+// One block per row j; each thread accumulates a fixed strided subset of
+// the row in FP64 and the block reduces in a fixed tree, so results depend
+// only on (l, m).  Per row: diagno sums (rho, u, v, u2+v2 with diagno's
+// per-node formulas), varm sums (rho, u, v with varm's formulas), the max
+// of u2+v2 and a non-finite counter.
+constexpr int ROWS_T = 256;
+constexpr int ROW_VALS = 9;
+
+template <class S, class C>
+__global__ __launch_bounds__(ROWS_T) void k_rows(Lat L, Phys<C> P, double *out)
+{
+    const int j = blockIdx.x + 1;
+    const C cte1 = (C)1.0;
+    double acc[ROW_VALS] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = threadIdx.x + 1; i <= L.l; i += ROWS_T) {
+        C a01 = get<S, C>(L, P, L.a, 0, i, j), a03 = get<S, C>(L, P, L.a, 1, i, j);
+        C a05 = get<S, C>(L, P, L.a, 2, i, j), a08 = get<S, C>(L, P, L.a, 3, i, j);
+        C a10 = get<S, C>(L, P, L.a, 4, i, j), a12 = get<S, C>(L, P, L.a, 5, i, j);
+        C a14 = get<S, C>(L, P, L.a, 6, i, j), a17 = get<S, C>(L, P, L.a, 7, i, j);
+        C a19 = get<S, C>(L, P, L.a, 8, i, j);
+        // diagno (SRC/diagno.F90:63-96)
+        C rho = (((+a01 + a03 + a10 + a12) + (+a05 + a08 + a14 + a17)) + a19) + cte1;
+        C rhoinv = (C)1.0 / rho;
+        C xj = ((a03 - a12) + (a01 - a10) + (a05 - a14)) * rhoinv;
+        C yj = ((a03 - a01) + (a12 - a10) + (a08 - a17)) * rhoinv;
+        C s = xj * xj + yj * yj;
+        acc[0] += (double)rho;
+        acc[1] += (double)xj;
+        acc[2] += (double)yj;
+        acc[3] += (double)s;
+        // varm (SRC/varm.F90:54-71)
+        C rv = (+a01 + a03 + a05 + a08 + a10 + a12 + a14 + a17 + a19) + cte1;
+        C rvi = (C)1.0 / rv;
+        acc[4] += (double)rv;
+        acc[5] += (double)((a01 + a03 + a05 - a10 - a12 - a14) * rvi);
+        acc[6] += (double)((a03 + a08 + a12 - a01 - a10 - a17) * rvi);
+        double sd = (double)s;
+        acc[7] = sd > acc[7] ? sd : acc[7];
+        if (!(sd <= 1.0e300) || !((double)rho <= 1.0e300) || !((double)rho >= -1.0e300)) acc[8] += 1.0;
+    }
+    __shared__ double sh[ROW_VALS][ROWS_T];
+#pragma unroll
+    for (int q = 0; q < ROW_VALS; q++) sh[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int w = ROWS_T / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int q = 0; q < ROW_VALS; q++) {
+                if (q == 7)
+                    sh[q][threadIdx.x] = fmax(sh[q][threadIdx.x], sh[q][threadIdx.x + w]);
+                else
+                    sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < ROW_VALS) out[(long long)(j - 1) * ROW_VALS + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// sums of the per-row values in a fixed order: out = {Srho, Su, Sv, Ss,
+// max s, nonfinite}
+__global__ __launch_bounds__(ROWS_T) void k_rows_final(const double *rows, int m, double *out)
+{
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int j = threadIdx.x; j < m; j += ROWS_T) {
+        const double *r = rows + (long long)j * ROW_VALS;
+        acc[0] += r[0];
+        acc[1] += r[1];
+        acc[2] += r[2];
+        acc[3] += r[3];
+        acc[4] = fmax(acc[4], r[7]);
+        acc[5] += r[8];
+    }
+    __shared__ double sh[6][ROWS_T];
+#pragma unroll
+    for (int q = 0; q < 6; q++) sh[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int w = ROWS_T / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                if (q == 4)
+                    sh[q][threadIdx.x] = fmax(sh[q][threadIdx.x], sh[q][threadIdx.x + w]);
+                else
+                    sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// prof_i (mode 0, row jc, n = l), prof_j (mode 1, column ic, n = m),
+// probe (mode 2, one node); out = u[n], v[n], rho[n]
+template <class S, class C>
+__global__ void k_line(Lat L, Phys<C> P, int mode, int c0, int c1, double *out)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = mode == 0 ? L.l : (mode == 1 ? L.m : 1);
+    if (t >= n) return;
+    int i, j;
+    if (mode == 0) { i = t + 1; j = c0; }
+    else if (mode == 1) { i = c0; j = t + 1; }
+    else { i = c0; j = c1; }
+    const C cte1 = (C)1.0;
+    C a01 = get<S, C>(L, P, L.a, 0, i, j), a03 = get<S, C>(L, P, L.a, 1, i, j);
+    C a05 = get<S, C>(L, P, L.a, 2, i, j), a08 = get<S, C>(L, P, L.a, 3, i, j);
+    C a10 = get<S, C>(L, P, L.a, 4, i, j), a12 = get<S, C>(L, P, L.a, 5, i, j);
+    C a14 = get<S, C>(L, P, L.a, 6, i, j), a17 = get<S, C>(L, P, L.a, 7, i, j);
+    C a19 = get<S, C>(L, P, L.a, 8, i, j);
+    C u, v, den;
+    if (mode == 1) {  // SRC/prof_j.F90:47-67
+        den = ((+a01 + a03 + a05 + a08 + a10 + a12 + a14 + a17) + a19) + cte1;
+        u = ((a01 - a10) + (a03 - a12) + (a05 - a14)) / den;
+        v = ((a03 - a01) + (a08 - a17) + (a12 - a10)) / den;
+    } else {  // SRC/prof_i.F90:47-63 and SRC/probe.F90:46-62
+        den = (+a01 + a03 + a05 + a08 + a10 + a12 + a14 + a17 + a19) + cte1;
+        u = (a01 + a03 + a05 - a10 - a12 - a14) / den;
+        v = (a03 + a08 + a12 - a01 - a10 - a17) / den;
+    }
+    out[t] = (double)u;
+    out[n + t] = (double)v;
+    out[2 * n + t] = (double)den;
+}
+
+// population k (0..8) <-> dense (l+2)*(m+2) array, x fastest
+template <class S>
+__global__ void k_pack(Lat L, const void *plane, S *dense)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > L.l + 1 || j > L.m + 1) return;
+    dense[(long long)j * (L.l + 2) + i] = elc<S>(plane, L.at(i, j));
+}
+template <class S>
+__global__ void k_unpack(Lat L, void *plane, const S *dense)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > L.l + 1 || j > L.m + 1) return;
+    el<S>(plane, L.at(i, j)) = dense[(long long)j * (L.l + 2) + i];
+}
+
+}  // namespace bgk

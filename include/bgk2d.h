@@ -1,0 +1,157 @@
+/*
+ * bgk2d.h -- C ABI of libbgk2d_hip.so, the MI355X (gfx950) implementation
+ * of BGK2D's time-step loop: D2Q9 BGK collision, ORIGINAL (two-pass) and
+ * FUSED (single-pass) streaming, every boundary variant, and the FP64
+ * diagnostics.
+ *
+ * Plain C types only.  Every entry point returns 0 on success or a
+ * negative BGK_E* code; bgk_last_error() gives the message.  One context is
+ * driven by one host thread; distinct contexts are independent and may be
+ * used from different threads.  All work of a context runs in order on its
+ * own HIP stream.  Coordinates are the reference's: 1-based interior
+ * (1..l, 1..m), halo ring at 0 and l+1 / m+1, x fastest.
+ *
+ * Reference interfaces replaced (the reference has no plugin registry; its
+ * "operators" are the parameterless subroutines below, all state living in
+ * module `storage`, SRC/storage.F90:25-140):
+ *   bgk_create          alloca + build_bcond + hencol  (SRC/alloca.F90:110-151,
+ *                       SRC/build.bcond.F90:36-45, SRC/hencol.F90:30-50)
+ *   bgk_build_obs       build_obs          (SRC/build.obs.F90:170-205)
+ *   bgk_set_obstacle    obs(:,:) from a file/caller (SRC/r.obs.F90)
+ *   bgk_init            init(opt)          (SRC/init.F90:83-137)
+ *   bgk_boundaries      boundaries         (SRC/boundaries.F90:23-64)
+ *   bgk_propagation     propagation/movef  (SRC/propagation.F90:31-45)
+ *   bgk_collision       collision(itime)   (SRC/collision.F90:37-41)
+ *   bgk_run             the loop body of SRC/bgk2d.F90:92-113 for n steps
+ *   bgk_diagno          diagno             (SRC/diagno.F90:63-101)
+ *   bgk_varm            varm               (SRC/varm.F90:48-79)
+ *   bgk_prof_i/_j       prof_i / prof_j    (SRC/prof_i.F90:47-63, SRC/prof_j.F90:47-67)
+ *   bgk_probe           probe              (SRC/probe.F90:46-62)
+ *   bgk_download/upload `!$omp target update` of a01..a19 and the records
+ *                       of save_raw/restore_raw (SRC/save_raw.F90:37-57,
+ *                       SRC/restore_raw.F90:41-51)
+ */
+#ifndef BGK2D_H
+#define BGK2D_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BGK_ABI_VERSION 1
+
+/* boundary variant: the compile-time -DLDC/-DPERIODIC/-DCHANNEL/-DINFLOW/
+ * -DCOUETTE/-DSTEP choice of SRC/boundaries.F90:31-56, made at run time  */
+enum bgk_bc {
+    BGK_BC_DRIVEN = 0,   /* bcond_driven   SRC/bcond.driven.F90:96-144   */
+    BGK_BC_PERIODIC = 1, /* bcond_periodic SRC/bcond.periodic.F90:60-107 */
+    BGK_BC_CHANNEL = 2,  /* bcond_channel  SRC/bcond.channel.F90:72-123  */
+    BGK_BC_INFLOW = 3,   /* bcond_inflow   SRC/bcond.inflow.F90:59-153   */
+    BGK_BC_COUETTE = 4,  /* bcond_couette  SRC/bcond.couette.F90:41-103  */
+    BGK_BC_STEP = 5      /* bcond_step     SRC/bcond.step.F90:55-254     */
+};
+
+/* precision: SRC/precision.F90:13-47 plus bf16 storage */
+enum bgk_prec {
+    BGK_PREC_SINGLE = 0, /* fp32 storage, fp32 compute                    */
+    BGK_PREC_DOUBLE = 1, /* fp64 storage, fp64 compute                    */
+    BGK_PREC_FP16 = 2,   /* IEEE half storage of scale*(f-w), fp32 compute
+                            (MIXED1 when scale = 1)                        */
+    BGK_PREC_MIXED2 = 3, /* fp32 storage, fp64 compute                    */
+    BGK_PREC_BF16 = 4    /* bfloat16 storage of scale*(f-w), fp32 compute */
+};
+
+enum bgk_impl { BGK_IMPL_ORIGINAL = 0, BGK_IMPL_FUSED = 1 };
+
+enum bgk_err {
+    BGK_OK = 0,
+    BGK_E_INVALID = -1,   /* bad argument / unsupported combination */
+    BGK_E_NOMEM = -2,     /* device or host allocation failed        */
+    BGK_E_HIP = -3,       /* HIP runtime error                       */
+    BGK_E_DIVERGED = -4,  /* NaN/Inf or |u| > 0.5 found by a check   */
+    BGK_E_STATE = -5      /* call out of order (e.g. run before init) */
+};
+
+typedef struct bgk_params {
+    int32_t lx, ly;       /* interior size (namelist lx, ly)                */
+    int32_t ipad, jpad;   /* namelist padding; accepted, layout is pitched  */
+    int32_t bc;           /* enum bgk_bc                                    */
+    int32_t obstacle;     /* 1: obstacle mask + bcond_obs (VKS build)       */
+    int32_t impl;         /* enum bgk_impl                                  */
+    int32_t prec;         /* enum bgk_prec                                  */
+    int32_t radius;       /* cylinder radius (namelist radius)              */
+    int32_t device;       /* HIP device ordinal                             */
+    int32_t mixed1_quirk; /* 1: zero u0 in hencol for FP16 as the MIXED1
+                             reference build does (SRC/hencol.F90:33-46)    */
+    int32_t reserved;
+    double svisc;         /* namelist svisc (kinematic viscosity)           */
+    double u0;            /* namelist u0 (lid speed, or channel force scale)*/
+    double scale;         /* FP16/BF16 storage scale (power of two, >= 1)   */
+} bgk_params_t;
+
+typedef struct bgk_ctx bgk_ctx_t;
+
+/* Context: allocates the two population sets in HBM (17 pitched planes),
+ * fills them with huge(mykind) as alloca does, and derives omega, fgrad
+ * and the lid speed in the compute kind exactly as hencol/build_bcond. */
+int bgk_create(const bgk_params_t *p, bgk_ctx_t **out);
+int bgk_destroy(bgk_ctx_t *ctx);
+const char *bgk_last_error(const bgk_ctx_t *ctx);
+int bgk_version(void);
+
+/* omega, fgrad, u00, force (= u00/6) in the compute kind, widened */
+int bgk_constants(const bgk_ctx_t *ctx, double out[4]);
+
+/* Cylinder of SRC/build.obs.F90:170-205 (centre 2l/5, m/2).  The mask
+ * must stay at least two nodes from every wall (else BGK_E_INVALID).    */
+int bgk_build_obs(bgk_ctx_t *ctx);
+/* Caller-supplied mask obs(1:l,1:m) as l*m int32, x fastest (1 = solid). */
+int bgk_set_obstacle(bgk_ctx_t *ctx, const int32_t *obs);
+int bgk_get_obstacle(bgk_ctx_t *ctx, int32_t *obs, int32_t bbox[4]);
+
+/* init(opt): shifted rest equilibrium, or for PERIODIC the Taylor-Green
+ * field u = A sin(2 pi x) cos(2 pi y), v = -A cos(2 pi x) sin(2 pi y).
+ * The reference hard-codes A = 0.1 (SRC/init.F90:109-110).             */
+int bgk_init(bgk_ctx_t *ctx, int32_t init_v, double amplitude);
+
+/* The three per-step phases of SRC/bgk2d.F90:100-102, each a kernel
+ * launch on the context stream with the reference's halo semantics.     */
+int bgk_boundaries(bgk_ctx_t *ctx);
+int bgk_propagation(bgk_ctx_t *ctx);
+int bgk_collision(bgk_ctx_t *ctx, int32_t itime);
+
+/* nsteps complete steps.  FUSED with a foldable boundary (all but STEP)
+ * runs one kernel per step with the boundary folded into the pull; the
+ * rest replay boundaries/propagation/collision.  Steps are captured into
+ * HIP graphs in chunks.  Interior results are identical to the per-phase
+ * entry points.                                                          */
+int bgk_run(bgk_ctx_t *ctx, int32_t nsteps);
+int bgk_sync(bgk_ctx_t *ctx);
+
+/* FP64 diagnostics, deterministic fixed-order reductions on the device.
+ * diagno: out = { <u>, <v>, <rho>, <u^2+v^2> } over 1..l x 1..m.        */
+int bgk_diagno(bgk_ctx_t *ctx, double out[4]);
+int bgk_varm(bgk_ctx_t *ctx, double *u, double *v, double *rho);           /* m each */
+int bgk_prof_i(bgk_ctx_t *ctx, int32_t jcoord, double *u, double *v, double *rho); /* l each */
+int bgk_prof_j(bgk_ctx_t *ctx, int32_t icoord, double *u, double *v, double *rho); /* m each */
+int bgk_probe(bgk_ctx_t *ctx, int32_t i0, int32_t j0, double out[3]);
+/* 0 if every node is finite with |u| <= umax, else BGK_E_DIVERGED */
+int bgk_check(bgk_ctx_t *ctx, double umax);
+
+/* Population k (0..8 = 01,03,05,08,10,12,14,17,19) of the current set,
+ * (l+2)*(m+2) elements of the storage kind (fp32, fp64, or the raw 16-bit
+ * pattern for FP16/BF16), x fastest, halos included.                    */
+int bgk_download(bgk_ctx_t *ctx, int32_t k, void *host);
+int bgk_upload(bgk_ctx_t *ctx, int32_t k, const void *host);
+int bgk_storage_bytes(const bgk_ctx_t *ctx);
+
+/* Stream and timing helpers for benchmarks: the context stream as an
+ * opaque handle, and the average duration (ms) of the main kernel of the
+ * last bgk_run measured with HIP events on that stream.                  */
+void *bgk_stream(bgk_ctx_t *ctx);
+int bgk_last_run_ms(const bgk_ctx_t *ctx, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -12,6 +12,7 @@
  * The per-precision routines live in orc_body.inc, included once for each
  * (storage kind, compute kind) pair of SRC/precision.F90:13-47.
  */
+#define _GNU_SOURCE
 #include "bgk_oracle.h"
 
 #include <math.h>
@@ -123,8 +124,7 @@ float orc_bf16_to_f32(uint16_t v)
 #define LDS(v) ((CT)(v))
 #define ENC(v) ((ST)(v))
 #define HUGEFILL ((float)2147483647)
-#define SINF sinf
-#define COSF cosf
+#define SINCOS sincosf
 #define SFX f32
 #include "orc_body.inc"
 #undef CT
@@ -132,8 +132,7 @@ float orc_bf16_to_f32(uint16_t v)
 #undef LDS
 #undef ENC
 #undef HUGEFILL
-#undef SINF
-#undef COSF
+#undef SINCOS
 #undef SFX
 
 /* DOUBLE: dp storage, dp compute */
@@ -142,8 +141,7 @@ float orc_bf16_to_f32(uint16_t v)
 #define LDS(v) ((CT)(v))
 #define ENC(v) ((ST)(v))
 #define HUGEFILL ((double)2147483647)
-#define SINF sin
-#define COSF cos
+#define SINCOS sincos
 #define SFX f64
 #include "orc_body.inc"
 #undef CT
@@ -151,8 +149,7 @@ float orc_bf16_to_f32(uint16_t v)
 #undef LDS
 #undef ENC
 #undef HUGEFILL
-#undef SINF
-#undef COSF
+#undef SINCOS
 #undef SFX
 
 /* MIXED2: sp storage, dp compute */
@@ -161,8 +158,7 @@ float orc_bf16_to_f32(uint16_t v)
 #define LDS(v) ((CT)(v))
 #define ENC(v) ((ST)(v))
 #define HUGEFILL ((float)2147483647)
-#define SINF sin
-#define COSF cos
+#define SINCOS sincos
 #define SFX mx2
 #include "orc_body.inc"
 #undef CT
@@ -170,8 +166,7 @@ float orc_bf16_to_f32(uint16_t v)
 #undef LDS
 #undef ENC
 #undef HUGEFILL
-#undef SINF
-#undef COSF
+#undef SINCOS
 #undef SFX
 
 /* MIXED1: IEEE half storage holding scale*(f - w), sp compute */
@@ -180,8 +175,7 @@ float orc_bf16_to_f32(uint16_t v)
 #define LDS(v) (orc_f16_to_f32(v) * (float)h->inv_scale)
 #define ENC(v) (orc_f32_to_f16((float)(v) * (float)h->scale))
 #define HUGEFILL ((uint16_t)0x7c00u)
-#define SINF sinf
-#define COSF cosf
+#define SINCOS sincosf
 #define SFX h16
 #include "orc_body.inc"
 #undef CT
@@ -189,8 +183,7 @@ float orc_bf16_to_f32(uint16_t v)
 #undef LDS
 #undef ENC
 #undef HUGEFILL
-#undef SINF
-#undef COSF
+#undef SINCOS
 #undef SFX
 
 /* BF16: bfloat16 storage holding scale*(f - w), sp compute */
@@ -199,8 +192,7 @@ float orc_bf16_to_f32(uint16_t v)
 #define LDS(v) (orc_bf16_to_f32(v) * (float)h->inv_scale)
 #define ENC(v) (orc_f32_to_bf16((float)(v) * (float)h->scale))
 #define HUGEFILL ((uint16_t)0x4f00u)
-#define SINF sinf
-#define COSF cosf
+#define SINCOS sincosf
 #define SFX b16
 #include "orc_body.inc"
 #undef CT
@@ -208,8 +200,7 @@ float orc_bf16_to_f32(uint16_t v)
 #undef LDS
 #undef ENC
 #undef HUGEFILL
-#undef SINF
-#undef COSF
+#undef SINCOS
 #undef SFX
 
 /* ------------------------------------------------------------- the API */

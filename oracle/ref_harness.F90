@@ -9,6 +9,7 @@
 !     alloca -> build_bcond -> [build_obs] -> init -> hencol
 !     do itime: boundaries -> propagation -> collision(itime)
 ! and dumps the current `a` populations (with halos) as raw stream bytes.
+! A dump file name of "-" skips the dump (timing runs).
 !
 ! usage: ref_harness lx ly svisc u0 nsteps radius dumpfile
 !   dump layout: int32 l, m, storage bytes, nsteps;
@@ -23,6 +24,7 @@ program ref_harness
   use real_kinds
   implicit none
   integer :: itime, nsteps, ios, esize
+  integer(8) :: c0, c1, crate
   character(len=256) :: arg, dumpfile
 
   call get_command_argument(1, arg); read(arg, *) lx
@@ -57,13 +59,19 @@ program ref_harness
   call init(0)
   call hencol
 
+  call system_clock(c0, crate)
   do itime = 1, nsteps
      call boundaries
      call propagation
      call collision(itime)
   end do
+  call system_clock(c1)
+  ! loop wall time and MLUPS as SRC/finalize.F90:60-61 defines it
+  write(6,'(a,f14.6,a,f14.3)') 'HARNESS loop_s=', real(c1-c0,8)/real(crate,8), &
+       ' mlups=', real(l,8)*real(m,8)*real(nsteps,8)/(real(c1-c0,8)/real(crate,8))/1.0d6
 
   esize = storage_size(a01(0,0)) / 8
+  if (trim(dumpfile) == '-') stop
   open(30, file=trim(dumpfile), access='stream', form='unformatted', &
        status='replace', iostat=ios)
   if (ios /= 0) stop 'cannot open dump file'

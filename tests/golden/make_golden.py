@@ -43,6 +43,12 @@ CASES = [
     ("SINGLE_ORIG_COU", 24, 16, 0.1, 0.1, 60, 16),
     ("SINGLE_FUSED_BFS", 60, 20, 0.1, 0.1, 50, 16),
     ("SINGLE_ORIG_BFS", 60, 20, 0.1, 0.1, 50, 16),
+    # wider grids: non-square Taylor-Green tables, two wavefronts per row
+    ("DOUBLE_FUSED_TGV@96x70", 96, 70, 0.02, 0.1, 3, 16),
+    ("SINGLE_FUSED_TGV@96x70", 96, 70, 0.02, 0.1, 3, 16),
+    ("MIXED1_FUSED_TGV@96x70", 96, 70, 0.02, 0.1, 3, 16),
+    ("SINGLE_FUSED_LDC@130x66", 130, 66, 0.02, 0.1, 20, 16),
+    ("SINGLE_FUSED_VKS@100x60", 100, 60, 0.04, 0.1, 30, 8),
 ]
 
 
@@ -63,7 +69,7 @@ def _read_text(path):
 
 
 def run_variant(name, lx, ly, svisc, u0, nsteps, radius):
-    exe = os.path.join(ROOT, "oracle", "_ref", "ref_" + name)
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_" + name.split("@")[0])
     with tempfile.TemporaryDirectory() as td:
         dump = os.path.join(td, "dump.bin")
         subprocess.run([exe, str(lx), str(ly), repr(svisc), repr(u0), str(nsteps),

@@ -1,0 +1,39 @@
+"""CPU checks of the drop-in boundary: the library loads and exports every
+entry point include/bgk2d.h declares (no compute calls)."""
+import ctypes
+import os
+import re
+
+import bgk2d_gpu_amd as pkg
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    txt = open(os.path.join(ROOT, "include", "bgk2d.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(bgk_\w+)\s*\(", txt)))
+
+
+def test_header_declares_expected_api():
+    names = declared()
+    assert set(pkg.EXPORTS) == set(names), set(pkg.EXPORTS) ^ set(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(pkg.build.LIB)
+    for n in declared():
+        assert hasattr(lib, n), n
+    assert lib.bgk_version() == 1
+
+
+def test_struct_layout_matches_header():
+    # bgk_params_t: 12 int32 then 3 doubles, no padding
+    assert ctypes.sizeof(pkg.solver.BgkParams) == 12 * 4 + 3 * 8
+
+
+def test_null_arguments_rejected_without_gpu():
+    L = pkg.load()
+    assert L.bgk_create(None, None) == -1
+    assert L.bgk_destroy(None) == -1
+    assert L.bgk_constants(None, None) == -1

@@ -1,0 +1,185 @@
+"""HIP path vs the reference and the oracle, through the C ABI.
+
+Bar: bit-exact populations (the arithmetic is the reference's, operation
+for operation, with no FMA contraction), FP64 diagnostics within 1e-12
+relative of the oracle's long-double sums.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from bgk2d_gpu_amd import Solver
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = np.load(os.path.join(HERE, "golden", "ref_fixtures.npz"))
+VARIANTS = sorted({k.split("/")[0] for k in FIX.files})
+GPU_PREC = {"SINGLE": "single", "DOUBLE": "double", "MIXED1": "fp16", "MIXED2": "mixed2"}
+ORC_PREC = {"single": "single", "double": "double", "fp16": "mixed1", "mixed2": "mixed2",
+            "bf16": "bf16"}
+
+
+def as_float(raw, prec):
+    if prec == "fp16":
+        return raw.view(np.float16).astype(np.float64)
+    if prec == "bf16":
+        return (raw.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    return raw.astype(np.float64)
+
+
+def same_bits(a, b, prec):
+    """bit equality, with every NaN equal to every NaN (x86 and gfx950
+    produce different default NaN patterns)"""
+    fa, fb = as_float(a, prec), as_float(b, prec)
+    nan = np.isnan(fa) & np.isnan(fb)
+    u = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[a.dtype.itemsize]
+    return (a.view(u) == b.view(u)) | nan
+
+
+def fixture_solver(name, impl_override=None):
+    p, impl, case = name.split("@")[0].split("_")
+    lx, ly, ns, rad = (int(x) for x in FIX[name + "/params"])
+    sv, u0 = (float(x) for x in FIX[name + "/fparams"])
+    bc, obst = po.CASE[case]
+    impl = impl_override or ("fused" if impl == "FUSED" else "original")
+    s = Solver(lx, ly, bc=bc, impl=impl, prec=GPU_PREC[p], svisc=sv, u0=u0, obstacle=obst,
+               radius=rad, mixed1_quirk=1 if p == "MIXED1" else 0)
+    return s, ns
+
+
+@pytest.mark.parametrize("name", VARIANTS)
+def test_reference_fixture_phases(name):
+    """boundaries -> propagation -> collision per step: whole planes,
+    halos included, equal the reference's"""
+    print(f"case {name} phases", flush=True)
+    s, ns = fixture_solver(name)
+    s.step_phases(ns)
+    got, ref = s.raw(), FIX[name + "/pops"]
+    ok = same_bits(got, ref, s.prec)
+    assert ok.all(), f"{int((~ok).sum())} differing elements"
+
+
+@pytest.mark.parametrize("name", VARIANTS)
+def test_reference_fixture_run(name):
+    """bgk_run (boundary folded into the fused pull where possible):
+    interior equals the reference's"""
+    print(f"case {name} run", flush=True)
+    s, ns = fixture_solver(name)
+    s.run(ns)
+    got, ref = s.raw(), FIX[name + "/pops"]
+    ok = same_bits(got[:, 1:-1, 1:-1], ref[:, 1:-1, 1:-1], s.prec)
+    assert ok.all(), f"{int((~ok).sum())} differing elements"
+
+
+CASES = [
+    # (bc, obstacle, lx, ly, svisc, u0, steps)
+    ("driven", 0, 131, 77, 0.05, 0.1, 40),
+    ("periodic", 0, 96, 70, 0.02, 0.1, 40),
+    ("channel", 0, 100, 33, 0.1, 0.08, 40),
+    ("inflow", 1, 150, 64, 0.04, 0.1, 40),
+    ("couette", 0, 70, 45, 0.08, 0.05, 40),
+    ("step", 0, 120, 40, 0.1, 0.1, 30),
+    ("driven", 1, 90, 66, 0.05, 0.1, 30),
+]
+
+
+@pytest.mark.parametrize("prec", ["single", "double", "fp16", "bf16", "mixed2"])
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}{'+obs' if c[1] else ''}")
+def test_against_oracle(case, prec):
+    bc, obst, lx, ly, sv, u0, ns = case
+    print(f"case {bc} obs={obst} {lx}x{ly} {prec}", flush=True)
+    scale = 8.0 if prec in ("fp16", "bf16") else 1.0
+    o = po.Oracle(lx, ly, bc=bc, impl="fused", prec=ORC_PREC[prec], svisc=sv, u0=u0,
+                  obstacle=obst, radius=9, scale=scale)
+    o.step(ns)
+    ref = o.raw()
+    for impl in ("fused", "original"):
+        s = Solver(lx, ly, bc=bc, impl=impl, prec=prec, svisc=sv, u0=u0, obstacle=obst,
+                   radius=9, scale=scale)
+        s.run(ns)
+        got = s.raw()
+        inner = (slice(None), slice(1, -1), slice(1, -1))
+        ok = same_bits(got[inner], ref[inner], prec)
+        if bc == "step" and impl == "original":
+            ok = ok[:, ly // 2:, :]  # FUSED reads never-written halos inside the step
+        assert ok.all(), f"{impl}: {int((~ok).sum())} differing elements"
+        if impl == "fused":
+            d, dref = s.diagno(), o.diagno(mode=0)
+            np.testing.assert_allclose(d, dref, rtol=1e-12, atol=1e-17)
+            for got_p, ref_p in ((s.varm(), o.varm()), (s.prof_j(lx // 2), o.prof_j(lx // 2)),
+                                 (s.prof_i(ly // 2), o.prof_i(ly // 2))):
+                for g, r in zip(got_p, ref_p):
+                    np.testing.assert_allclose(g, r, rtol=1e-12, atol=1e-17)
+            np.testing.assert_array_equal(s.probe(lx // 2, ly // 2), o.probe(lx // 2, ly // 2))
+        s.close()
+
+
+def test_phases_equal_run_fused_driven_bitexact():
+    """the folded kernel and the three-phase path agree bit for bit"""
+    a = Solver(257, 129, bc="driven", impl="fused", svisc=0.02)
+    b = Solver(257, 129, bc="driven", impl="fused", svisc=0.02)
+    a.run(101)
+    b.step_phases(101)
+    assert np.array_equal(a.raw()[:, 1:-1, 1:-1], b.raw()[:, 1:-1, 1:-1])
+
+
+def test_upload_download_roundtrip_and_restart():
+    """save_raw/restore_raw semantics: a state uploaded into a fresh
+    context continues bit-identically"""
+    a = Solver(64, 48, bc="periodic", svisc=0.05)
+    a.run(20)
+    snap = a.raw()
+    a.run(15)
+    b = Solver(64, 48, bc="periodic", svisc=0.05, init=False)
+    b.set_raw(snap)
+    assert np.array_equal(b.raw(), snap)
+    b.run(15)
+    assert np.array_equal(a.raw()[:, 1:-1, 1:-1], b.raw()[:, 1:-1, 1:-1])
+
+
+def test_divergence_check():
+    s = Solver(32, 32, bc="periodic", svisc=0.05)
+    s.run(5)
+    assert s.check(0.5)
+    raw = s.raw()
+    raw[2, 10, 10] = np.nan
+    s.set_raw(raw)
+    assert not s.check(0.5)
+
+
+def test_invalid_arguments_fail_loudly():
+    from bgk2d_gpu_amd import BgkError
+    with pytest.raises(BgkError):
+        Solver(1, 10)
+    with pytest.raises(BgkError):
+        Solver(10, 10, bc="step")  # stepx = l/20 = 0
+    with pytest.raises(BgkError):
+        Solver(32, 32, prec="fp16", scale=3.0)  # scale must be a power of two
+    s = Solver(40, 40, init=False)
+    with pytest.raises(BgkError):
+        s.run(1)  # before init
+    mask = np.zeros((40, 40), dtype=np.int32)
+    mask[0, 5] = 1  # touches the first interior row
+    with pytest.raises(BgkError):
+        s.set_obstacle(mask)
+
+
+def test_cavity_re100_256_against_reference_profile_and_ghia():
+    """256^2, Re=100 (nu=0.256), 50,000 steps: the configuration of
+    UTIL/bgk.256.R100.input; compare the centre-line profile with the
+    reference-held TEST/DrivenCavity/prof_j.0000.dat last block and Ghia"""
+    s = Solver(256, 256, bc="driven", impl="fused", prec="single", svisc=0.256, u0=0.1)
+    s.run(50000)
+    u, v, r = s.prof_j(128)
+    ref = np.loadtxt(os.path.join(HERE, "golden", "prof_j_R100_256_t50000.dat"), comments="#")
+    np.testing.assert_allclose(u, ref[:, 1], atol=2e-3 * 0.1 + 1e-4)
+    gy = np.loadtxt(os.path.join(HERE, "golden", "ghia_y.dat"))
+    um = 0.5 * (s.prof_j(128)[0] + s.prof_j(129)[0]) / 0.1
+    y = (np.arange(1, 257) - 0.5) / 256
+    sel = (gy[:, 0] > 0.02) & (gy[:, 0] < 0.98)
+    assert np.max(np.abs(np.interp(gy[sel, 0], y, um) - gy[sel, 1])) < 0.015
+    d = s.diagno()
+    assert abs(d[2] - 1.0) < 1e-4
