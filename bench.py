@@ -27,9 +27,13 @@ BYTES_PER_LUP = {"single": 72, "fp16": 36, "bf16": 36, "double": 144, "mixed2": 
 
 
 def gpu_solve(n, prec, bc, steps, warmup, svisc, u0):
+    """FP16/BF16 run with the configuration that meets the accuracy bar:
+    scale 1024, deviation form, stochastic rounding"""
     from bgk2d_gpu_amd import Solver
+    narrow = prec in ("fp16", "bf16")
     s = Solver(n, n, bc=bc, impl="fused", prec=prec, svisc=svisc, u0=u0,
-               scale=1024.0 if prec in ("fp16", "bf16") else 1.0, amplitude=0.01)
+               scale=1024.0 if narrow else 1.0, amplitude=0.01, form=1 if narrow else 0,
+               rounding=1 if narrow else 0)
     if warmup:
         s.run(warmup)
     s.sync()
@@ -71,6 +75,84 @@ def cpu_baseline(budget_s=20.0):
             "sample": f"4096x4096 LDC Re=1e4, {steps} steps, oracle C restatement"}
 
 
+CX = (1, 1, 1, 0, -1, -1, -1, 0, 0)
+CY = (-1, 1, 0, 1, -1, 1, 0, -1, 0)
+
+
+def velocity(solver, scale):
+    """u, v of the interior from the downloaded populations (float64)"""
+    import numpy as np
+    raw = solver.raw()[:, 1:-1, 1:-1]
+    if solver.prec == "fp16":
+        f = raw.view(np.float16).astype(np.float32).astype(np.float64) / scale
+    elif solver.prec == "bf16":
+        f = (raw.astype(np.uint32) << 16).view(np.float32).astype(np.float64) / scale
+    else:
+        f = raw.astype(np.float64)
+    del raw
+    rho = f.sum(axis=0) + 1.0
+    u = sum(c * f[k] for k, c in enumerate(CX) if c) / rho
+    v = sum(c * f[k] for k, c in enumerate(CY) if c) / rho
+    return u, v
+
+
+def precision_study(n=8192, steps=5000, amp=0.01, svisc=0.1, init_v=0):
+    """Taylor-Green n^2, tau = 0.8, u0 = 0.01: velocity relative L2 error of
+    every storage mode against the FP64 run, and against the analytic
+    field u(t) = u(0) exp(-2 nu k^2 t)."""
+    import numpy as np
+    from bgk2d_gpu_amd import Solver
+    res = {"workload": f"Taylor-Green {n}x{n}, tau=0.8 (svisc={svisc}), u0={amp}, {steps} steps, "
+                       + ("rho=1 start" if init_v == 0 else "pressure-consistent density start"),
+           "metric": "relative L2 = ||u-u_ref||_2/||u_ref||_2 over all nodes, both components"}
+    s = Solver(n, n, bc="periodic", impl="fused", prec="double", svisc=svisc, amplitude=amp,
+               init_v=init_v)
+    s.run(steps)
+    s.sync()
+    ur, vr = velocity(s, 1.0)
+    s.close()
+    norm = np.sqrt((ur * ur + vr * vr).sum())
+    x = (np.arange(1, n + 1) - 0.5) / n
+    k = 2 * np.pi
+    decay = np.exp(-svisc * 2 * (k / n) ** 2 * steps)  # nu |k|^2 t, |k|^2 = 2 (2 pi/n)^2
+    ua = amp * np.sin(k * x)[None, :] * np.cos(k * x)[:, None] * decay
+    va = -amp * np.cos(k * x)[None, :] * np.sin(k * x)[:, None] * decay
+    anorm = np.sqrt((ua * ua + va * va).sum())
+    res["double_vs_analytic"] = float(np.sqrt(((ur - ua) ** 2 + (vr - va) ** 2).sum()) / anorm)
+    for prec, scale, form, rnd in (("single", 1.0, 0, 0), ("single", 1.0, 1, 0),
+                                   ("fp16", 1.0, 0, 0), ("fp16", 1024.0, 1, 0),
+                                   ("fp16", 1024.0, 1, 1), ("bf16", 1024.0, 1, 1)):
+        s = Solver(n, n, bc="periodic", impl="fused", prec=prec, svisc=svisc, amplitude=amp,
+                   scale=scale, form=form, init_v=init_v, rounding=rnd)
+        s.run(steps)
+        s.sync()
+        u, v = velocity(s, scale)
+        s.close()
+        err = float(np.sqrt(((u - ur) ** 2 + (v - vr) ** 2).sum()) / norm)
+        tag = f"{prec}_scale{int(scale)}_form{form}_{'sr' if rnd else 'rne'}"
+        res[tag + "_vs_double"] = err
+        erra = float(np.sqrt(((u - ua) ** 2 + (v - va) ** 2).sum()) / anorm)
+        res[tag + "_vs_analytic"] = erra
+        print(f"# {tag}: rel L2 {err:.3e} vs double, {erra:.3e} vs analytic",
+              file=sys.stderr, flush=True)
+        del u, v
+    return res
+
+
+def copy_calib():
+    """Plain device copies of a known size, for scaling the FETCH_SIZE /
+    WRITE_SIZE counters to bytes."""
+    import torch
+    a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calib_bytes_each_way": a.numel() * 4, "copies": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,7 +160,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp16-steps", type=int, default=2000)
+    ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="only run the FP32/FP16/BF16 vs FP64 Taylor-Green error study")
+    ap.add_argument("--study-n", type=int, default=8192)
+    ap.add_argument("--study-steps", type=int, default=5000)
+    ap.add_argument("--study-init", type=int, default=0)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run 5 torch copies of 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calib:
+        copy_calib()
+        return
+    if args.precision_study:
+        print(json.dumps(precision_study(args.study_n, args.study_steps, init_v=args.study_init)))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(each request or slab shards per GPU; see DESIGN.md)", file=sys.stderr)
@@ -90,12 +186,13 @@ def main():
     kern_s = r32["kernel_ms"] / 1e3
     achieved = BYTES_PER_LUP["single"] * n * n / kern_s / 1e9
 
-    # FP16 shifted storage: Taylor-Green 8192^2 (tau = 0.8, u0 = 0.01) vs FP32
-    n3 = 8192
-    t16 = gpu_solve(n3, "fp16", "periodic", args.fp16_steps, 50, 0.1, 0.0)
-    t32 = gpu_solve(n3, "single", "periodic", args.fp16_steps, 50, 0.1, 0.0)
-    ml16 = n3 * n3 * args.fp16_steps / t16["wall"] / 1e6
-    ml32 = n3 * n3 * args.fp16_steps / t32["wall"] / 1e6
+    if not args.no_fp16:
+        # FP16 shifted storage: Taylor-Green 8192^2 (tau = 0.8, u0 = 0.01) vs FP32
+        n3 = 8192
+        t16 = gpu_solve(n3, "fp16", "periodic", args.fp16_steps, 50, 0.1, 0.0)
+        t32 = gpu_solve(n3, "single", "periodic", args.fp16_steps, 50, 0.1, 0.0)
+        ml16 = n3 * n3 * args.fp16_steps / t16["wall"] / 1e6
+        ml32 = n3 * n3 * args.fp16_steps / t32["wall"] / 1e6
 
     out = {
         "metric": METRIC,
@@ -117,13 +214,15 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "kernel_ms_per_step": r32["kernel_ms"],
         "healthy": bool(r32["ok"]),
-        "fp16_store": {"workload": "Taylor-Green 8192x8192 periodic, tau=0.8, u0=0.01, "
-                                   f"{args.fp16_steps} steps, scale=1024",
-                       "mlups_fp16": round(ml16, 1), "mlups_fp32": round(ml32, 1),
-                       "ratio": round(ml16 / ml32, 3),
-                       "hbm_frac_fp16": round(36 * n3 * n3 / (t16["kernel_ms"] / 1e3) / 1e9
-                                              / HBM_PEAK_GBS, 4)},
     }
+    if not args.no_fp16:
+        out["fp16_store"] = {"workload": "Taylor-Green 8192x8192 periodic, tau=0.8, u0=0.01, "
+                                         f"{args.fp16_steps} steps, scale=1024, deviation form, "
+                                         "stochastic rounding",
+                             "mlups_fp16": round(ml16, 1), "mlups_fp32": round(ml32, 1),
+                             "ratio": round(ml16 / ml32, 3),
+                             "hbm_frac_fp16": round(36 * n3 * n3 / (t16["kernel_ms"] / 1e3) / 1e9
+                                                    / HBM_PEAK_GBS, 4)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

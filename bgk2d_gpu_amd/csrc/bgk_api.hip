@@ -77,24 +77,6 @@ int esize_of(int prec)
     }
 }
 
-// host-side storage round trip used for the inflow constants
-float host_rt_f16(float v, float sc, float inv) { return (float)(_Float16)(v * sc) * inv; }
-float host_rt_bf16(float v, float sc, float inv)
-{
-    float t = v * sc;
-    uint32_t x;
-    memcpy(&x, &t, 4);
-    if ((x & 0x7fffffffu) > 0x7f800000u) {
-        x = ((x >> 16) | 0x40u) << 16;
-    } else {
-        x += 0x7fffu + ((x >> 16) & 1u);
-        x = (x >> 16) << 16;
-    }
-    float r;
-    memcpy(&r, &x, 4);
-    return r * inv;
-}
-
 // hencol + build_bcond in the compute kind C (SRC/hencol.F90:30-50,
 // SRC/build.bcond.F90:36-45), and the stored inflow equilibria of
 // SRC/bcond.inflow.F90:101-113 with u_inflow = 0.1 (a default-real literal).
@@ -121,14 +103,9 @@ void derive(bgk_ctx *c)
     const C cvsq = xj * xj + yj * yj;
     const C cxq[3] = {eq_d<C>(xj - yj, cvsq), eq_d<C>(xj + yj, cvsq), eq_d<C>(xj, cvsq)};
     const C wq[3] = {K<C>::p2, K<C>::p2, K<C>::p1};
-    for (int q = 0; q < 3; q++) {
-        C t = (C)1.0 * wq[q] * ((C)0.0 + cxq[q]);  // crho*p*(cte1+cx)
-        double r = (double)t;
-        if (is_fp16(p.prec)) r = host_rt_f16((float)t, (float)c->scale, (float)(1.0 / c->scale));
-        if (is_bf16(p.prec)) r = host_rt_bf16((float)t, (float)c->scale, (float)(1.0 / c->scale));
-        if (p.prec == BGK_PREC_MIXED2) r = (double)(float)t;
-        c->inflow[q] = r;
-    }
+    // crho*p*(cte1+cx) in the compute kind; each stored copy is rounded to
+    // the storage kind where it is stored (x = 0, row j)
+    for (int q = 0; q < 3; q++) c->inflow[q] = (double)((C)1.0 * wq[q] * ((C)0.0 + cxq[q]));
 }
 
 template <class C>
@@ -142,6 +119,8 @@ Phys<C> phys(const bgk_ctx *c)
     P.inv = (C)(1.0 / c->scale);
     for (int q = 0; q < 3; q++) P.inflow[q] = (C)c->inflow[q];
     P.channel_force = c->p.bc == BGK_BC_CHANNEL ? 1 : 0;
+    P.form = c->p.form;
+    P.rounding = (is_fp16(c->p.prec) || is_bf16(c->p.prec)) ? c->p.rounding : 0;
     return P;
 }
 
@@ -159,6 +138,7 @@ Lat lat(const bgk_ctx *c)
     L.m = c->m;
     L.pitch = c->pitch;
     L.off = c->off;
+    L.rs = c->pitch;
     L.imin = c->imin;
     L.imax = c->imax;
     L.jmin = c->jmin;
@@ -284,11 +264,53 @@ int launch_step(bgk_ctx *c)
     return r;
 }
 
+// Steps are replayed from HIP graphs of GRAPH_STEPS launches each (an even
+// count, so the FUSED set parity is the same before and after a replay);
+// one graph per starting parity, built on first use.
+constexpr int GRAPH_STEPS = 32;
+
+template <class S, class C>
+int graph_for(bgk_ctx *c, hipGraphExec_t *out)
+{
+    const long long key = c->cur;
+    auto it = c->graphs.find(key);
+    if (it != c->graphs.end()) {
+        *out = it->second;
+        return BGK_OK;
+    }
+    const int cur0 = c->cur;
+    HIPCHK(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+    int r = BGK_OK;
+    for (int t = 0; t < GRAPH_STEPS && r == BGK_OK; t++) r = launch_step<S, C>(c);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->s, &g);
+    c->cur = cur0;
+    if (r != BGK_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+    }
+    if (e != hipSuccess) return fail(c, BGK_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(c, BGK_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    c->graphs[key] = x;
+    *out = x;
+    return BGK_OK;
+}
+
 template <class S, class C>
 int run_t(bgk_ctx *c, int nsteps)
 {
     HIPCHK(c, hipEventRecord(c->ev0, c->s));
-    for (int t = 0; t < nsteps; t++) {
+    int t = 0;
+    if (nsteps >= GRAPH_STEPS) {
+        hipGraphExec_t g;
+        int r = graph_for<S, C>(c, &g);
+        if (r != BGK_OK) return r;
+        for (; t + GRAPH_STEPS <= nsteps; t += GRAPH_STEPS) HIPCHK(c, hipGraphLaunch(g, c->s));
+    }
+    for (; t < nsteps; t++) {
         int r = launch_step<S, C>(c);
         if (r != BGK_OK) return r;
     }
@@ -311,7 +333,7 @@ template <>
 void host_sincos<double>(double arg, double *s, double *c) { sincos(arg, s, c); }
 
 template <class S, class C>
-int init_t(bgk_ctx *c, double amp)
+int init_t(bgk_ctx *c, double amp, int init_v)
 {
     const bool tgv = c->p.bc == BGK_BC_PERIODIC;
     const int nx = c->l + 2, ny = c->m + 2;
@@ -336,7 +358,7 @@ int init_t(bgk_ctx *c, double amp)
     Lat L = lat(c);
     Phys<C> P = phys<C>(c);
     hipLaunchKernelGGL((k_init<S, C>), grid2(nx, ny, B2), B2, 0, c->s, L, P, d, d + nx, d + 2 * nx,
-                       d + 2 * nx + ny, amp, tgv ? 1 : 0);
+                       d + 2 * nx + ny, amp, tgv ? (init_v == 1 ? 2 : 1) : 0);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->s));
     HIPCHK(c, hipFree(d));
@@ -432,6 +454,8 @@ int bgk_create(const bgk_params_t *p, bgk_ctx_t **out)
         return BGK_E_INVALID;
     if (p->bc == BGK_BC_STEP && p->lx < 20) return BGK_E_INVALID;  // stepx = l/20 must be >= 1
     if (!(p->svisc > 0.0)) return BGK_E_INVALID;
+    if (p->form != 0 && p->form != 1) return BGK_E_INVALID;
+    if (p->rounding != 0 && p->rounding != 1) return BGK_E_INVALID;
     bgk_ctx *c = new bgk_ctx();
     c->p = *p;
     c->l = p->lx;
@@ -629,7 +653,7 @@ int bgk_init(bgk_ctx_t *c, int32_t init_v, double amplitude)
     int r = check_ctx(c);
     if (r) return r;
     if (init_v < 0 || init_v > 5) return fail(c, BGK_E_INVALID, "Initial condition out of range[0,5] %d", init_v);
-    r = DISPATCH(c, init_t, amplitude);
+    r = DISPATCH(c, init_t, amplitude, (int)init_v);
     if (r == BGK_OK) c->inited = true;
     return r;
 }

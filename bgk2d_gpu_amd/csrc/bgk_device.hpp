@@ -37,6 +37,34 @@ struct bf16_t {
     uint16_t bits;
 };
 
+// Stochastic rounding of FP16/BF16 stores (opt-in): the bits below the
+// target precision get a pseudo-random offset before truncation, which makes
+// the rounding unbiased; with round-to-nearest, slowly varying fields lose
+// their sub-ulp per-step changes systematically.  The offset is a hash of
+// the value's own bits and its position (i, j, population), so results are
+// deterministic and independent of launch grouping.
+__host__ __device__ inline uint32_t mix32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+// one full mix per node (shared by its nine stores), a Weyl step per population
+__host__ __device__ inline uint32_t salt_of(int i, int j, int k)
+{
+    return mix32((uint32_t)i * 0x9E3779B1u + (uint32_t)j * 0x85EBCA77u) + (uint32_t)k * 0x9E3779B9u;
+}
+// per-store offset: one 24-bit multiply (full rate on CDNA) of the low bits
+// of value^salt, which change from step to step, then an xorshift
+__host__ __device__ inline uint32_t sr_hash(uint32_t bits, uint32_t salt)
+{
+    uint32_t h = ((bits ^ salt) & 0xffffffu) * 0xB5297Au;
+    return h ^ (h >> 15);
+}
+
 template <class S, class C>
 struct Codec;
 
@@ -44,26 +72,37 @@ template <>
 struct Codec<float, float> {
     using raw = float;
     __device__ __forceinline__ static float dec(float v, float) { return v; }
-    __device__ __forceinline__ static float enc(float v, float) { return v; }
+    __device__ __forceinline__ static float enc(float v, float, int, uint32_t) { return v; }
 };
 template <>
 struct Codec<double, double> {
     using raw = double;
     __device__ __forceinline__ static double dec(double v, double) { return v; }
-    __device__ __forceinline__ static double enc(double v, double) { return v; }
+    __device__ __forceinline__ static double enc(double v, double, int, uint32_t) { return v; }
 };
 template <>
 struct Codec<float, double> {  // MIXED2
     using raw = float;
     __device__ __forceinline__ static double dec(float v, double) { return (double)v; }
-    __device__ __forceinline__ static float enc(double v, double) { return (float)v; }
+    __device__ __forceinline__ static float enc(double v, double, int, uint32_t) { return (float)v; }
 };
 template <>
 struct Codec<_Float16, float> {
     using raw = _Float16;
     // dec/enc take the scale factor that applies: inv_scale / scale
     __device__ __forceinline__ static float dec(_Float16 v, float inv) { return (float)v * inv; }
-    __device__ __forceinline__ static _Float16 enc(float v, float sc) { return (_Float16)(v * sc); }
+    __device__ __forceinline__ static _Float16 enc(float v, float sc, int sr, uint32_t salt)
+    {
+        float x = v * sc;
+        if (sr) {
+            uint32_t b = __float_as_uint(x), a = b & 0x7fffffffu;
+            if (a >= 0x38800000u && a < 0x477fe000u) {  // normal half range
+                b += (sr_hash(b, salt) >> 8) & 0x1fffu;
+                x = __uint_as_float(b & ~0x1fffu);
+            }
+        }
+        return (_Float16)x;
+    }
 };
 template <>
 struct Codec<bf16_t, float> {
@@ -72,12 +111,15 @@ struct Codec<bf16_t, float> {
     {
         return __uint_as_float((uint32_t)v.bits << 16) * inv;
     }
-    __device__ __forceinline__ static bf16_t enc(float v, float sc)
+    __device__ __forceinline__ static bf16_t enc(float v, float sc, int sr, uint32_t salt)
     {
         uint32_t x = __float_as_uint(v * sc);
         bf16_t r;
         if ((x & 0x7fffffffu) > 0x7f800000u) {
             r.bits = (uint16_t)((x >> 16) | 0x40u);
+        } else if (sr && (x & 0x7fffffffu) < 0x7f7f0000u) {
+            x += (sr_hash(x, salt) >> 8) & 0xffffu;
+            r.bits = (uint16_t)(x >> 16);
         } else {
             x += 0x7fffu + ((x >> 16) & 1u);
             r.bits = (uint16_t)(x >> 16);
@@ -94,9 +136,10 @@ struct Lat {
     const uint8_t *obs;  // l*m solid mask (x fastest) or null
     int l, m;
     int pitch, off;
+    long long rs;                // elements from row j to row j+1 of one population
     int imin, imax, jmin, jmax;  // obstacle bounding box (1-based)
     int stepx, stepy;
-    __device__ __forceinline__ long long at(int i, int j) const { return (long long)off + (long long)j * pitch + i; }
+    __device__ __forceinline__ long long at(int i, int j) const { return (long long)off + (long long)j * rs + i; }
 };
 
 template <class C>
@@ -105,6 +148,8 @@ struct Phys {
     C sc, inv;              // storage scale and its inverse (FP16/BF16)
     C inflow[3];            // stored-and-decoded inflow equilibria 01,03,05
     int channel_force;      // 1: CHANNEL body force in collision
+    int form;               // 0: reference operation order; 1: deviation form
+    int rounding;           // FP16/BF16 stores: 0 nearest-even, 1 stochastic
 };
 
 // ---------------------------------------------------------- arithmetic
@@ -128,7 +173,11 @@ __device__ __forceinline__ void relax(const Phys<C> &P, const C x[9], C o[9])
     const C rf = K<C>::rf, qf = K<C>::qf, p0 = K<C>::p0, p1 = K<C>::p1, p2 = K<C>::p2;
     const C x01 = x[0], x03 = x[1], x05 = x[2], x08 = x[3], x10 = x[4];
     const C x12 = x[5], x14 = x[6], x17 = x[7], x19 = x[8];
-    C rho = (((x01 + x03 + x10 + x12) + (x05 + x08 + x14 + x17)) + x19) + cte1;
+    // deviation form: drho = sum of the shifted populations is kept apart
+    // from the 1, and the equilibria use w*drho instead of (rho*w - w), so
+    // the O(u^2) density deviation is not rounded against 1
+    const C drho = ((x01 + x03 + x10 + x12) + (x05 + x08 + x14 + x17)) + x19;
+    C rho = P.form ? (C)1.0 + drho : drho + cte1;
     C rhoinv = (C)1.0 / rho;
     C vx = ((x03 - x12) + (x01 - x10) + (x05 - x14)) * rhoinv;
     C vy = ((x03 - x01) + (x12 - x10) + (x08 - x17)) * rhoinv;
@@ -145,15 +194,19 @@ __device__ __forceinline__ void relax(const Phys<C> &P, const C x[9], C o[9])
     vxpy = rf * vxpy;
     vxmy = rf * vxmy;
     C rp0 = rho * p0, rp1 = rho * p1, rp2 = rho * p2;
-    C e01 = rp2 * (+vxmy + qxmy) + cte1 * (rp2 - p2);
-    C e03 = rp2 * (+vxpy + qxpy) + cte1 * (rp2 - p2);
-    C e05 = rp1 * (+vx + qx) + cte1 * (rp1 - p1);
-    C e08 = rp1 * (+vy + qy) + cte1 * (rp1 - p1);
-    C e10 = rp2 * (-vxpy + qxpy) + cte1 * (rp2 - p2);
-    C e12 = rp2 * (-vxmy + qxmy) + cte1 * (rp2 - p2);
-    C e14 = rp1 * (-vx + qx) + cte1 * (rp1 - p1);
-    C e17 = rp1 * (-vy + qy) + cte1 * (rp1 - p1);
-    C e19 = rp0 * (+q0) + cte1 * (rp0 - p0);
+    // the density term cte1*(rp - p) of the reference, or p*drho
+    const C d0 = P.form ? p0 * drho : cte1 * (rp0 - p0);
+    const C d1 = P.form ? p1 * drho : cte1 * (rp1 - p1);
+    const C d2 = P.form ? p2 * drho : cte1 * (rp2 - p2);
+    C e01 = rp2 * (+vxmy + qxmy) + d2;
+    C e03 = rp2 * (+vxpy + qxpy) + d2;
+    C e05 = rp1 * (+vx + qx) + d1;
+    C e08 = rp1 * (+vy + qy) + d1;
+    C e10 = rp2 * (-vxpy + qxpy) + d2;
+    C e12 = rp2 * (-vxmy + qxmy) + d2;
+    C e14 = rp1 * (-vx + qx) + d1;
+    C e17 = rp1 * (-vy + qy) + d1;
+    C e19 = rp0 * (+q0) + d0;
     C forcex = (C)0.0, forcey = (C)0.0;
     if (P.channel_force) forcex = P.fgrad * rho;
     const C om = P.omega;

@@ -34,23 +34,41 @@ __device__ __forceinline__ C get(const Lat &L, const Phys<C> &P, void *const *se
     return CodecT<S, C>::dec(elc<S>(p, L.at(i, j)), P.inv);
 }
 template <class S, class C>
+__device__ __forceinline__ typename CodecT<S, C>::raw encode(const Phys<C> &P, C v, int i, int j, int k)
+{
+    return CodecT<S, C>::enc(v, P.sc, P.rounding, P.rounding ? salt_of(i, j, k) : 0u);
+}
+template <class S, class C>
 __device__ __forceinline__ void put(const Lat &L, const Phys<C> &P, void *const *set, int k, int i, int j, C v)
 {
     void *p = (k == 8) ? L.r : set[k];
-    el<S>(p, L.at(i, j)) = CodecT<S, C>::enc(v, P.sc);
+    el<S>(p, L.at(i, j)) = encode<S, C>(P, v, i, j, k);
 }
+// streaming store of a result population: written once per step and not
+// read again by this launch, so it bypasses cache retention
+__device__ __forceinline__ void st_nt(float *p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_nt(double *p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_nt(_Float16 *p, _Float16 v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_nt(bf16_t *p, bf16_t v) { __builtin_nontemporal_store(v.bits, &p->bits); }
+template <class S, class C>
+__device__ __forceinline__ void put_nt(const Lat &L, const Phys<C> &P, void *const *set, int k, int i, int j, C v)
+{
+    void *p = (k == 8) ? L.r : set[k];
+    st_nt(&el<S>(p, L.at(i, j)), encode<S, C>(P, v, i, j, k));
+}
+
 // raw copy of a stored element (the reference's `a(x) = a(y)` assignments)
 template <class S>
 __device__ __forceinline__ void cpy(const Lat &L, void *dst, int di, int dj, const void *src, int si, int sj)
 {
     el<S>(dst, L.at(di, dj)) = elc<S>(src, L.at(si, sj));
 }
-// storage round trip of a compute-kind value (a value stored to a halo by
-// a boundary and read back by the pull)
+// storage round trip of a compute-kind value stored at halo node (i, j) as
+// population k by a boundary and read back by the pull
 template <class S, class C>
-__device__ __forceinline__ C rt(const Phys<C> &P, C v)
+__device__ __forceinline__ C rt(const Phys<C> &P, C v, int i, int j, int k)
 {
-    return CodecT<S, C>::dec(CodecT<S, C>::enc(v, P.sc), P.inv);
+    return CodecT<S, C>::dec(encode<S, C>(P, v, i, j, k), P.inv);
 }
 
 // ------------------------------------------------------------ fill/init
@@ -76,6 +94,23 @@ __global__ void k_init(Lat L, Phys<C> P, const C *tsx, const C *tcx, const C *ts
     if (tgv) {
         xj = (C)(amp * (double)tsx[i] * (double)tcy[j]);
         yj = (C)(-(amp * (double)tcx[i] * (double)tsy[j]));
+    }
+    if (tgv == 2) {
+        // Taylor-Green with the density of its pressure field,
+        // rho = 1 + (3 A^2/4)(cos 4 pi x + cos 4 pi y), so that no acoustic
+        // transient is launched (an extension; the reference starts at rho = 1)
+        const double c2x = (double)tcx[i] * (double)tcx[i] - (double)tsx[i] * (double)tsx[i];
+        const double c2y = (double)tcy[j] * (double)tcy[j] - (double)tsy[j] * (double)tsy[j];
+        const double drho = 0.75 * amp * amp * (c2x + c2y);
+        const double u = (double)xj, v = (double)yj, usq = u * u + v * v;
+        const double w[9] = {1.0 / 36, 1.0 / 36, 1.0 / 9, 1.0 / 9, 1.0 / 36, 1.0 / 36, 1.0 / 9, 1.0 / 9, 4.0 / 9};
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const double cu = cx_of(k) * u + cy_of(k) * v;
+            const double f = w[k] * drho + w[k] * (1.0 + drho) * (3.0 * cu + 4.5 * cu * cu - 1.5 * usq);
+            put<S, C>(L, P, L.a, k, i, j, (C)f);
+        }
+        return;
     }
     const C rf = K<C>::rf, qf = K<C>::qf, p0 = K<C>::p0, p1 = K<C>::p1, p2 = K<C>::p2;
     const C crho = (C)1.0, cte1 = (C)0.0;
@@ -286,7 +321,7 @@ __global__ void k_col(Lat L, Phys<C> P)
     x[8] = get<S, C>(L, P, L.a, 8, i, j);
     relax<C>(P, x, o);
 #pragma unroll
-    for (int k = 0; k < 9; k++) put<S, C>(L, P, L.a, k, i, j, o[k]);
+    for (int k = 0; k < 9; k++) put_nt<S, C>(L, P, L.a, k, i, j, o[k]);
 }
 
 template <class S, class C>
@@ -301,8 +336,8 @@ __global__ void k_colmc(Lat L, Phys<C> P)
     x[8] = get<S, C>(L, P, L.a, 8, i, j);
     relax<C>(P, x, o);
 #pragma unroll
-    for (int k = 0; k < 8; k++) put<S, C>(L, P, L.b, k, i, j, o[k]);
-    put<S, C>(L, P, L.a, 8, i, j, o[8]);
+    for (int k = 0; k < 8; k++) put_nt<S, C>(L, P, L.b, k, i, j, o[k]);
+    put_nt<S, C>(L, P, L.a, 8, i, j, o[8]);
 }
 
 // ------------------------------------------------ fused, boundary folded
@@ -332,7 +367,7 @@ __device__ C folded(const Lat &L, const Phys<C> &P, int k, int i, int j, int si,
     }
     if (BC == BK_INFLOW) {
         sj = sj < 1 ? m : (sj > m ? 1 : sj);
-        if (si < 1) return P.inflow[k];  // only pops 01,03,05 come from x = 0
+        if (si < 1) return rt<S, C>(P, P.inflow[k], 0, sj, k);  // pops 01,03,05 from x = 0
         if (si > l) {
             // outflow equilibrium from the momentum of (l, sj)
             const C rf = K<C>::rf, qf = K<C>::qf, p1 = K<C>::p1, p2 = K<C>::p2;
@@ -351,7 +386,7 @@ __device__ C folded(const Lat &L, const Phys<C> &P, int k, int i, int j, int si,
                 v = crho * p2 * (cte1 + (rf * (-xj + yj) + qf * ((C)3.0 * (xj - yj) * (xj - yj) - cvsq)));
             else
                 v = crho * p1 * (cte1 + (rf * (-xj) + qf * ((C)3.0 * (xj) * (xj)-cvsq)));
-            return rt<S, C>(P, v);
+            return rt<S, C>(P, v, l + 1, sj, k);
         }
         return get<S, C>(L, P, L.a, k, si, sj);
     }
@@ -361,7 +396,7 @@ __device__ C folded(const Lat &L, const Phys<C> &P, int k, int i, int j, int si,
         C own = get<S, C>(L, P, L.a, o, i, j);
         const int c = cx_of(k);
         if (c == 0) return own;
-        return rt<S, C>(P, c > 0 ? own + P.force : own - P.force);
+        return rt<S, C>(P, c > 0 ? own + P.force : own - P.force, si, sj, k);
     }
     if (sj < 1 || sj > m) return get<S, C>(L, P, L.a, o, i, j);
     if (BC == BK_DRIVEN) return get<S, C>(L, P, L.a, o, i, j);
@@ -400,8 +435,8 @@ __global__ __launch_bounds__(256) void k_fused(Lat L, Phys<C> P)
     }
     relax<C>(P, x, o);
 #pragma unroll
-    for (int k = 0; k < 8; k++) put<S, C>(L, P, L.b, k, i, j, o[k]);
-    put<S, C>(L, P, L.a, 8, i, j, o[8]);
+    for (int k = 0; k < 8; k++) put_nt<S, C>(L, P, L.b, k, i, j, o[k]);
+    put_nt<S, C>(L, P, L.a, 8, i, j, o[8]);
 }
 
 // ------------------------------------------------------------ diagnostics

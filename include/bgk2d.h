@@ -83,7 +83,13 @@ typedef struct bgk_params {
     int32_t device;       /* HIP device ordinal                             */
     int32_t mixed1_quirk; /* 1: zero u0 in hencol for FP16 as the MIXED1
                              reference build does (SRC/hencol.F90:33-46)    */
-    int32_t reserved;
+    int32_t form;         /* 0: the reference's operation order (bit-exact
+                             with its builds); 1: deviation form, the
+                             density deviation kept apart from 1           */
+    int32_t rounding;     /* FP16/BF16 stores: 0 round to nearest even (the
+                             reference's MIXED1); 1 deterministic stochastic
+                             rounding (hash of value and position)          */
+    int32_t reserved;     /* must be 0                                      */
     double svisc;         /* namelist svisc (kinematic viscosity)           */
     double u0;            /* namelist u0 (lid speed, or channel force scale)*/
     double scale;         /* FP16/BF16 storage scale (power of two, >= 1)   */
@@ -111,7 +117,10 @@ int bgk_get_obstacle(bgk_ctx_t *ctx, int32_t *obs, int32_t bbox[4]);
 
 /* init(opt): shifted rest equilibrium, or for PERIODIC the Taylor-Green
  * field u = A sin(2 pi x) cos(2 pi y), v = -A cos(2 pi x) sin(2 pi y).
- * The reference hard-codes A = 0.1 (SRC/init.F90:109-110).             */
+ * The reference hard-codes A = 0.1 (SRC/init.F90:109-110) and only
+ * range-checks opt (0..5); here opt = 1 with PERIODIC also sets the
+ * density of the vortex's pressure field, rho = 1 + (3A^2/4)(cos 4 pi x +
+ * cos 4 pi y), which launches no acoustic transient.                    */
 int bgk_init(bgk_ctx_t *ctx, int32_t init_v, double amplitude);
 
 /* The three per-step phases of SRC/bgk2d.F90:100-102, each a kernel

@@ -116,13 +116,61 @@ float orc_bf16_to_f32(uint16_t v)
     return r;
 }
 
+/* Stochastic rounding of half/bf16 stores: a hash of the value's low bits
+ * and its position (i, j, k) is added below the kept bits before
+ * truncation (the same rule as the HIP codec).                          */
+static uint32_t mix32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+static uint32_t salt_of(int i, int j, int k)
+{
+    return mix32((uint32_t)i * 0x9E3779B1u + (uint32_t)j * 0x85EBCA77u) + (uint32_t)k * 0x9E3779B9u;
+}
+static uint32_t sr_hash(uint32_t bits, uint32_t salt)
+{
+    uint32_t h = ((bits ^ salt) & 0xffffffu) * 0xB5297Au;
+    return h ^ (h >> 15);
+}
+static uint16_t enc_f16(float v, float sc, int sr, int i, int j, int k)
+{
+    float x = v * sc;
+    if (sr) {
+        uint32_t b;
+        memcpy(&b, &x, 4);
+        uint32_t a = b & 0x7fffffffu;
+        if (a >= 0x38800000u && a < 0x477fe000u) {
+            b += (sr_hash(b, salt_of(i, j, k)) >> 8) & 0x1fffu;
+            b &= ~0x1fffu;
+            memcpy(&x, &b, 4);
+        }
+    }
+    return orc_f32_to_f16(x);
+}
+static uint16_t enc_bf16(float v, float sc, int sr, int i, int j, int k)
+{
+    float x = v * sc;
+    uint32_t b;
+    memcpy(&b, &x, 4);
+    if (sr && (b & 0x7fffffffu) < 0x7f7f0000u) {
+        b += (sr_hash(b, salt_of(i, j, k)) >> 8) & 0xffffu;
+        return (uint16_t)(b >> 16);
+    }
+    return orc_f32_to_bf16(x);
+}
+
 /* ------------------------------------------------- precision instances */
 
 /* SINGLE: sp storage, sp compute */
 #define CT float
 #define ST float
 #define LDS(v) ((CT)(v))
-#define ENC(v) ((ST)(v))
+#define ENC(v, i, j, k) ((ST)(v))
 #define HUGEFILL ((float)2147483647)
 #define SINCOS sincosf
 #define SFX f32
@@ -139,7 +187,7 @@ float orc_bf16_to_f32(uint16_t v)
 #define CT double
 #define ST double
 #define LDS(v) ((CT)(v))
-#define ENC(v) ((ST)(v))
+#define ENC(v, i, j, k) ((ST)(v))
 #define HUGEFILL ((double)2147483647)
 #define SINCOS sincos
 #define SFX f64
@@ -156,7 +204,7 @@ float orc_bf16_to_f32(uint16_t v)
 #define CT double
 #define ST float
 #define LDS(v) ((CT)(v))
-#define ENC(v) ((ST)(v))
+#define ENC(v, i, j, k) ((ST)(v))
 #define HUGEFILL ((float)2147483647)
 #define SINCOS sincos
 #define SFX mx2
@@ -173,7 +221,7 @@ float orc_bf16_to_f32(uint16_t v)
 #define CT float
 #define ST uint16_t
 #define LDS(v) (orc_f16_to_f32(v) * (float)h->inv_scale)
-#define ENC(v) (orc_f32_to_f16((float)(v) * (float)h->scale))
+#define ENC(v, i, j, k) enc_f16((float)(v), (float)h->scale, h->p.rounding, (i), (j), (k))
 #define HUGEFILL ((uint16_t)0x7c00u)
 #define SINCOS sincosf
 #define SFX h16
@@ -190,7 +238,7 @@ float orc_bf16_to_f32(uint16_t v)
 #define CT float
 #define ST uint16_t
 #define LDS(v) (orc_bf16_to_f32(v) * (float)h->inv_scale)
-#define ENC(v) (orc_f32_to_bf16((float)(v) * (float)h->scale))
+#define ENC(v, i, j, k) enc_bf16((float)(v), (float)h->scale, h->p.rounding, (i), (j), (k))
 #define HUGEFILL ((uint16_t)0x4f00u)
 #define SINCOS sincosf
 #define SFX b16

@@ -42,6 +42,8 @@ typedef struct {
     int32_t init_v;        /* namelist init_v, range-checked only         */
     int32_t mixed1_quirk;  /* 1: hencol zeroes u0 for MIXED1 (reference)  */
     int32_t tgv;           /* 1: Taylor-Green initial field (PERIODIC)    */
+    int32_t form;          /* 0: reference operation order; 1: deviation  */
+    int32_t rounding;      /* half/bf16 stores: 0 nearest, 1 stochastic   */
     double svisc, u0;      /* namelist values                             */
     double amplitude;      /* TGV amplitude (reference hard-codes 0.1)    */
     double scale;          /* storage scale for half/bf16 (power of two)  */

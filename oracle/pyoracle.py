@@ -27,6 +27,7 @@ class OrcParams(C.Structure):
                 ("obstacle", C.c_int32), ("impl", C.c_int32), ("prec", C.c_int32),
                 ("radius", C.c_int32), ("init_v", C.c_int32),
                 ("mixed1_quirk", C.c_int32), ("tgv", C.c_int32),
+                ("form", C.c_int32), ("rounding", C.c_int32),
                 ("svisc", C.c_double), ("u0", C.c_double),
                 ("amplitude", C.c_double), ("scale", C.c_double)]
 
@@ -86,12 +87,12 @@ class Oracle:
 
     def __init__(self, lx, ly, bc="driven", impl="fused", prec="single", svisc=0.1,
                  u0=0.1, obstacle=0, radius=16, amplitude=0.1, scale=1.0,
-                 mixed1_quirk=0, init_v=0, tgv=None):
+                 mixed1_quirk=0, init_v=0, tgv=None, form=0, rounding=0):
         L = lib()
         if tgv is None:
             tgv = 1 if bc == "periodic" else 0
         self.p = OrcParams(lx, ly, BC[bc], obstacle, IMPL[impl], PREC[prec], radius,
-                           init_v, mixed1_quirk, tgv, svisc, u0, amplitude, scale)
+                           init_v, mixed1_quirk, tgv, form, rounding, svisc, u0, amplitude, scale)
         self.prec = prec
         self.l, self.m = lx, ly
         self.h = L.orc_create(C.byref(self.p))

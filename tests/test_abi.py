@@ -28,8 +28,8 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_layout_matches_header():
-    # bgk_params_t: 12 int32 then 3 doubles, no padding
-    assert ctypes.sizeof(pkg.solver.BgkParams) == 12 * 4 + 3 * 8
+    # bgk_params_t: 14 int32 then 3 doubles, no padding
+    assert ctypes.sizeof(pkg.solver.BgkParams) == 14 * 4 + 3 * 8
 
 
 def test_null_arguments_rejected_without_gpu():

@@ -1,0 +1,116 @@
+"""The Fortran driver (bgk2d_hip): bgk.input semantics, output formats and
+save/restore, on top of the C ABI through ISO_C_BINDING."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from bgk2d_gpu_amd import build
+
+INPUT = """&parameters
+lx = {lx}
+ly = {ly}
+svisc={svisc}
+u0={u0}
+itfin={itfin}
+ivtim=50000
+isignal={isignal}
+itsave={itsave}
+icheck={icheck}
+irestart={irestart}
+init_v=0
+ipad=0        /
+&gpu
+case_name='{case}'
+precision='{prec}'
+implementation='{impl}'
+phases={phases}
+/
+"""
+
+
+def write_input(d, **kw):
+    args = dict(lx=64, ly=48, svisc=0.05, u0=0.1, itfin=300, isignal=100, itsave=300,
+                icheck=100, irestart=0, case="LDC", prec="SINGLE", impl="FUSED", phases=0)
+    args.update(kw)
+    with open(os.path.join(d, "bgk.input"), "w") as f:
+        f.write(INPUT.format(**args))
+    return args
+
+
+def run_driver(d, timeout=120):
+    return subprocess.run([build.DRIVER], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                          text=True, timeout=timeout)
+
+
+def read_save(path, dtype, l, m):
+    """Fortran unformatted sequential file: 4-byte length markers per record"""
+    raw = open(path, "rb").read()
+    pos, recs = 0, []
+    while pos < len(raw):
+        n = int(np.frombuffer(raw[pos:pos + 4], dtype=np.int32)[0])
+        recs.append(raw[pos + 4:pos + 4 + n])
+        pos += n + 8
+    itime = int(np.frombuffer(recs[0], dtype=np.int32)[0])
+    pops = np.stack([np.frombuffer(r, dtype=dtype).reshape(m + 2, l + 2) for r in recs[1:10]])
+    return itime, pops
+
+
+def test_driver_built_and_linked():
+    assert os.path.exists(build.DRIVER)
+    out = subprocess.run(["ldd", build.DRIVER], stdout=subprocess.PIPE, text=True).stdout
+    assert "libbgk2d_hip.so" in out and "not found" not in out.split("libbgk2d_hip.so")[1].split("\n")[0]
+
+
+def test_driver_rejects_unknown_case_before_gpu(tmp_path):
+    write_input(str(tmp_path), case="XYZ")
+    r = run_driver(str(tmp_path))
+    assert r.returncode != 0 and "unknown case_name" in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("phases", [0, 1])
+@pytest.mark.parametrize("case,prec,impl", [("LDC", "SINGLE", "FUSED"), ("TGV", "DOUBLE", "ORIGINAL"),
+                                            ("VKS", "SINGLE", "FUSED"), ("POF", "MIXED1", "FUSED")])
+def test_driver_matches_oracle(tmp_path, case, prec, impl, phases):
+    d = str(tmp_path)
+    a = write_input(d, case=case, prec=prec, impl=impl, phases=phases, lx=80, ly=40,
+                    svisc=0.05, itfin=300, itsave=300)
+    r = run_driver(d)
+    assert r.returncode == 0, r.stdout
+    bc, obst = po.CASE[case]
+    oprec = {"SINGLE": "single", "DOUBLE": "double", "MIXED1": "mixed1"}[prec]
+    o = po.Oracle(80, 40, bc=bc, impl=impl.lower(), prec=oprec, svisc=0.05, u0=0.1,
+                  obstacle=obst, radius=16 if not obst else 16, mixed1_quirk=int(prec == "MIXED1"))
+    o.step(300)
+    it, pops = read_save(os.path.join(d, "save.00000300.bin"), po.STORE_DTYPE[oprec], 80, 40)
+    assert it == 300
+    ref = o.raw()
+    inner = (slice(None), slice(1, -1), slice(1, -1))
+    assert np.array_equal(pops[inner].view(np.uint8), ref[inner].view(np.uint8))
+    if phases == 1:
+        # the per-phase path reproduces the reference's halos too
+        assert np.array_equal(pops.view(np.uint8), ref.view(np.uint8))
+    diag = np.loadtxt(os.path.join(d, "diagno.dat"))
+    np.testing.assert_allclose(diag[-1, 1:], o.diagno(), rtol=1e-5, atol=1e-12)
+    assert diag[-1, 0] == 300
+    perf = open(os.path.join(d, "bgk.perf")).read()
+    assert "Mlups" in perf
+
+
+@pytest.mark.gpu
+def test_driver_restart_continues_identically(tmp_path):
+    d1, d2 = str(tmp_path / "a"), str(tmp_path / "b")
+    os.makedirs(d1)
+    os.makedirs(d2)
+    write_input(d1, itfin=400, itsave=200, icheck=200)
+    assert run_driver(d1).returncode == 0
+    os.symlink(os.path.join(d1, "save.00000200.bin"), os.path.join(d2, "restore.bin"))
+    write_input(d2, itfin=400, itsave=200, icheck=200, irestart=1)
+    r = run_driver(d2)
+    assert r.returncode == 0, r.stdout
+    _, a = read_save(os.path.join(d1, "save.00000400.bin"), np.float32, 64, 48)
+    _, b = read_save(os.path.join(d2, "save.00000400.bin"), np.float32, 64, 48)
+    assert np.array_equal(a[:, 1:-1, 1:-1], b[:, 1:-1, 1:-1])

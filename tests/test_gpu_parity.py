@@ -117,6 +117,43 @@ def test_against_oracle(case, prec):
         s.close()
 
 
+OPTION_CASES = [
+    # (bc, obstacle, prec, form, rounding, init_v)
+    ("periodic", 0, "fp16", 1, 1, 1),
+    ("periodic", 0, "bf16", 1, 1, 0),
+    ("driven", 0, "fp16", 0, 1, 0),
+    ("inflow", 1, "fp16", 1, 1, 0),
+    ("channel", 0, "bf16", 0, 1, 0),
+    ("couette", 0, "single", 1, 0, 0),
+    ("periodic", 0, "double", 1, 0, 1),
+    ("step", 0, "fp16", 1, 1, 0),
+]
+
+
+@pytest.mark.parametrize("case", OPTION_CASES, ids=lambda c: "-".join(map(str, c)))
+def test_options_against_oracle(case):
+    """deviation form, stochastic FP16/BF16 rounding and the pressure-
+    consistent Taylor-Green start, bit-exact against the oracle"""
+    bc, obst, prec, form, rounding, init_v = case
+    lx, ly, ns = 130, 70, 37
+    o = po.Oracle(lx, ly, bc=bc, impl="fused", prec=ORC_PREC[prec], svisc=0.05, u0=0.05,
+                  obstacle=obst, radius=9, scale=64.0, form=form, rounding=rounding,
+                  init_v=init_v, amplitude=0.05)
+    o.step(ns)
+    ref = o.raw()
+    for impl in ("fused", "original"):
+        s = Solver(lx, ly, bc=bc, impl=impl, prec=prec, svisc=0.05, u0=0.05, obstacle=obst,
+                   radius=9, scale=64.0, form=form, rounding=rounding, init_v=init_v,
+                   amplitude=0.05)
+        s.run(ns)
+        got = s.raw()
+        ok = same_bits(got[:, 1:-1, 1:-1], ref[:, 1:-1, 1:-1], prec)
+        if bc == "step" and impl == "original":
+            ok = ok[:, ly // 2:, :]
+        assert ok.all(), f"{impl}: {int((~ok).sum())} differing elements"
+        s.close()
+
+
 def test_phases_equal_run_fused_driven_bitexact():
     """the folded kernel and the three-phase path agree bit for bit"""
     a = Solver(257, 129, bc="driven", impl="fused", svisc=0.02)
@@ -124,6 +161,21 @@ def test_phases_equal_run_fused_driven_bitexact():
     a.run(101)
     b.step_phases(101)
     assert np.array_equal(a.raw()[:, 1:-1, 1:-1], b.raw()[:, 1:-1, 1:-1])
+
+
+def test_graph_replays_match_oracle_with_eager_work_between():
+    """bgk_run replays HIP graphs of 32 steps; replays interleaved with
+    diagnostics and eager steps stay bit-exact with the oracle"""
+    for bc, impl in (("channel", "fused"), ("step", "fused"), ("driven", "original")):
+        s = Solver(96, 40, bc=bc, impl=impl, svisc=0.05, u0=0.05)
+        o = po.Oracle(96, 40, bc=bc, impl=impl, svisc=0.05, u0=0.05)
+        for n in (64, 5, 33, 96, 1):
+            s.run(n)
+            o.step(n)
+            s.diagno()
+            got, ref = s.raw(), o.raw()
+            ok = same_bits(got[:, 1:-1, 1:-1], ref[:, 1:-1, 1:-1], "single")
+            assert ok.all(), f"{bc}/{impl} after {n}: {int((~ok).sum())} differ"
 
 
 def test_upload_download_roundtrip_and_restart():

@@ -279,3 +279,70 @@ def test_cavity_ghia_re100():
     assert np.max(np.abs(np.interp(gy[sel, 0], y, u) - gy[sel, 1])) < 0.03
     sel = (gx[:, 0] > 0.02) & (gx[:, 0] < 0.98)
     assert np.max(np.abs(np.interp(gx[sel, 0], x, v) - gx[sel, 1])) < 0.03
+
+
+# ---------------------------------------------- options beyond the reference
+
+def test_deviation_form_is_textbook_bgk():
+    """form 1 keeps the density deviation apart from 1; the step is still the
+    textbook BGK update (checked in double against the numpy code)"""
+    l, m = 12, 10
+    o = po.Oracle(l, m, bc="periodic", impl="fused", prec="double", svisc=0.1, tgv=0, form=1)
+    f0 = _random_state(o, seed=3)
+    omega = o.constants()["omega"]
+    pulled = _pull_periodic(f0[:, 1:-1, 1:-1])
+    rho, ux, uy = moments(pulled)
+    expect = pulled - omega * (pulled - feq_shifted(rho, ux, uy))
+    o.step(1)
+    np.testing.assert_allclose(o.pops()[:, 1:-1, 1:-1], expect, rtol=0, atol=2e-16)
+
+
+def test_stochastic_rounding_brackets_and_is_unbiased():
+    """the same fp32 collision results stored by nearest-even and by
+    stochastic rounding differ by at most one half-precision step, and the
+    stochastic errors average out over positions (unbiased)"""
+    l, m = 64, 64
+    shape = (m + 2, l + 2)
+    rng = np.random.default_rng(11)
+    feq = feq_shifted(1 + 1e-3 * rng.standard_normal(shape), 0.01 + 1e-3 * rng.standard_normal(shape),
+                      1e-3 * rng.standard_normal(shape))
+    raw_in = feq.astype(np.float32).astype(np.float16).view(np.uint16)
+    out = {}
+    for r in (0, 1):
+        # omega = 1: the stored result is the equilibrium of the pulled state
+        o = po.Oracle(l, m, bc="periodic", impl="fused", prec="mixed1", svisc=1 / 6,
+                      rounding=r, tgv=0)
+        o.set_raw(raw_in)
+        o.step(1)
+        out[r] = o.raw()[:, 1:-1, 1:-1].view(np.float16).astype(np.float64)
+    rn, sr = out[0], out[1]
+    ulp = np.abs(np.spacing(rn.astype(np.float16))).astype(np.float64)
+    assert np.all(np.abs(sr - rn) <= ulp * 1.0001)
+    assert (sr != rn).mean() > 0.2  # the rule does change values
+    dec_in = raw_in.view(np.float16).astype(np.float64)
+    rho, ux, uy = moments(_pull_periodic(dec_in[:, 1:-1, 1:-1]))
+    exact = feq_shifted(rho, ux, uy)
+    bias_sr = np.abs((sr - exact).mean(axis=(1, 2)))
+    assert np.all(bias_sr < 0.05 * ulp.mean(axis=(1, 2)))
+
+
+def test_pressure_consistent_taylor_green_matches_analytic():
+    """init_v = 1: density of the vortex's pressure field; the solution then
+    follows the decaying vortex without an acoustic transient (positions use
+    the default-real pi of SRC/init.F90, hence the 1e-10 on the density)"""
+    L, nu, A, T = 48, 0.05, 0.01, 200
+    o = po.Oracle(L, L, bc="periodic", impl="fused", prec="double", svisc=nu, amplitude=A,
+                  init_v=1)
+    f = o.pops()[:, 1:-1, 1:-1]
+    rho, ux, uy = moments(f)
+    x = (np.arange(1, L + 1) - 0.5) / L
+    k = 2 * np.pi
+    np.testing.assert_allclose(rho - 1, 0.75 * A * A * (np.cos(2 * k * x)[None, :] +
+                                                          np.cos(2 * k * x)[:, None]), atol=1e-10)
+    o.step(T)
+    rho, ux, uy = moments(o.pops()[:, 1:-1, 1:-1])
+    dec = np.exp(-nu * 2 * (k / L) ** 2 * T)
+    ua = A * np.sin(k * x)[None, :] * np.cos(k * x)[:, None] * dec
+    err = np.sqrt(((ux - ua) ** 2).sum() / (ua ** 2).sum())
+    # O(k^2) lattice error of the decay rate at L = 48 (exponent 0.34)
+    assert err < 4e-3
