@@ -231,7 +231,12 @@ void launch_fused_t(bgk_ctx *c)
 {
     Lat L = lat(c);
     Phys<C> P = phys<C>(c);
-    hipLaunchKernelGGL((k_fused<S, C, BC, OBS>), grid2(c->l, c->m, B2), B2, 0, c->s, L, P);
+    if constexpr (sizeof(S) == 2) {
+        // 16-bit storage: two nodes per thread
+        hipLaunchKernelGGL((k_fused2<S, C, BC, OBS>), grid2((c->l + 1) / 2, c->m, B2), B2, 0, c->s, L, P);
+    } else {
+        hipLaunchKernelGGL((k_fused<S, C, BC, OBS>), grid2(c->l, c->m, B2), B2, 0, c->s, L, P);
+    }
     c->cur = 1 - c->cur;
 }
 
@@ -474,7 +479,9 @@ int bgk_create(const bgk_params_t *p, bgk_ctx_t **out)
     c->esize = esize_of(p->prec);
     c->A = 128 / c->esize;
     c->off = c->A - 1;
-    c->pitch = ((c->A + c->l + 1) + c->A - 1) / c->A * c->A;
+    // a row holds nodes 0..l+1 from element off = A-1, plus two elements of
+    // slack for the paired 16-bit accesses at the end of a row
+    c->pitch = ((c->A + c->l + 3) + c->A - 1) / c->A * c->A;
     size_t pe = (size_t)(c->m + 2) * (size_t)c->pitch;
     const size_t align = 256 / c->esize;
     c->plane_elems = (pe + align - 1) / align * align;

@@ -43,25 +43,25 @@ struct bf16_t {
 // their sub-ulp per-step changes systematically.  The offset is a hash of
 // the value's own bits and its position (i, j, population), so results are
 // deterministic and independent of launch grouping.
-__host__ __device__ inline uint32_t mix32(uint32_t x)
+// position salt: 24-bit multiplies (full rate on CDNA) and a Weyl step per
+// population; the per-store hash below does the mixing
+__host__ __device__ inline uint32_t mul24(uint32_t a, uint32_t b)
 {
-    x ^= x >> 16;
-    x *= 0x7feb352du;
-    x ^= x >> 15;
-    x *= 0x846ca68bu;
-    x ^= x >> 16;
-    return x;
+#ifdef __HIP_DEVICE_COMPILE__
+    return __umul24(a, b);
+#else
+    return (a & 0xffffffu) * (b & 0xffffffu);
+#endif
 }
-// one full mix per node (shared by its nine stores), a Weyl step per population
 __host__ __device__ inline uint32_t salt_of(int i, int j, int k)
 {
-    return mix32((uint32_t)i * 0x9E3779B1u + (uint32_t)j * 0x85EBCA77u) + (uint32_t)k * 0x9E3779B9u;
+    return mul24((uint32_t)i, 0x9E3779u) + mul24((uint32_t)j, 0x85EBCAu) + (uint32_t)k * 0x9E3779B9u;
 }
 // per-store offset: one 24-bit multiply (full rate on CDNA) of the low bits
 // of value^salt, which change from step to step, then an xorshift
 __host__ __device__ inline uint32_t sr_hash(uint32_t bits, uint32_t salt)
 {
-    uint32_t h = ((bits ^ salt) & 0xffffffu) * 0xB5297Au;
+    uint32_t h = mul24(bits ^ salt, 0xB5297Au);
     return h ^ (h >> 15);
 }
 

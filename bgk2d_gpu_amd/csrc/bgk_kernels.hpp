@@ -404,18 +404,12 @@ __device__ C folded(const Lat &L, const Phys<C> &P, int k, int i, int j, int si,
     return get<S, C>(L, P, L.a, k, si, sj);
 }
 
+// Replace the pulled values whose source is a halo node (folded boundary)
+// or a solid node (bcond_obs), in that order, as the reference applies the
+// wall boundary before the obstacle one.
 template <class S, class C, int BC, bool OBS>
-__global__ __launch_bounds__(256) void k_fused(Lat L, Phys<C> P)
+__device__ __forceinline__ void fix_pulls(const Lat &L, const Phys<C> &P, int i, int j, C x[9])
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    if (i > L.l || j > L.m) return;
-    C x[9], o[9];
-    // plain pulls; the halo ring is allocated, so edge nodes read it too
-    // and then replace those values below
-#pragma unroll
-    for (int k = 0; k < 8; k++) x[k] = get<S, C>(L, P, L.a, k, i - cx_of(k), j - cy_of(k));
-    x[8] = get<S, C>(L, P, L.a, 8, i, j);
     if (i == 1 || i == L.l || j == 1 || j == L.m) {
 #pragma unroll
         for (int k = 0; k < 8; k++) {
@@ -433,10 +427,89 @@ __global__ __launch_bounds__(256) void k_fused(Lat L, Phys<C> P)
             if (solid(L, si, sj) && !(me && before_pop(k))) x[k] = get<S, C>(L, P, L.a, opp_of(k), i, j);
         }
     }
+}
+
+// Addressing of the hot kernel: one 32-bit byte offset per node, and per
+// population a uniform base pointer that already includes the population's
+// pull shift, so every load and store is scalar base + vector offset with
+// no per-access address arithmetic (planes are smaller than 4 GiB).
+template <class S>
+__device__ __forceinline__ const char *pull_base(const Lat &L, const void *plane, int k)
+{
+    return (const char *)plane + ((long long)L.off - (long long)cy_of(k) * L.rs - cx_of(k)) * (long long)sizeof(S);
+}
+
+template <class S, class C, int BC, bool OBS>
+__global__ __launch_bounds__(256) void k_fused(Lat L, Phys<C> P)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > L.l || j > L.m) return;
+    using R = typename CodecT<S, C>::raw;
+    const uint32_t bo = ((uint32_t)j * (uint32_t)L.rs + (uint32_t)i) * (uint32_t)sizeof(S);
+    C x[9], o[9];
+    // plain pulls; the halo ring is allocated, so edge nodes read it too
+    // and then replace those values below
+#pragma unroll
+    for (int k = 0; k < 8; k++) x[k] = CodecT<S, C>::dec(*(const R *)(pull_base<S>(L, L.a[k], k) + bo), P.inv);
+    x[8] = CodecT<S, C>::dec(*(const R *)(pull_base<S>(L, L.r, 8) + bo), P.inv);
+    fix_pulls<S, C, BC, OBS>(L, P, i, j, x);
     relax<C>(P, x, o);
 #pragma unroll
-    for (int k = 0; k < 8; k++) put_nt<S, C>(L, P, L.b, k, i, j, o[k]);
-    put_nt<S, C>(L, P, L.a, 8, i, j, o[8]);
+    for (int k = 0; k < 8; k++) st_nt((R *)(pull_base<S>(L, L.b[k], 8) + bo), encode<S, C>(P, o[k], i, j, k));
+    st_nt((R *)(pull_base<S>(L, L.r, 8) + bo), encode<S, C>(P, o[8], i, j, 8));
+}
+
+// The same step for 16-bit storage, two nodes (i0, i0+1) per thread with
+// i0 odd: each population is moved with 32-bit accesses, halving the
+// memory instructions of the 16-bit kinds.  Node i0 starts a dword (the
+// first interior node is 128-byte aligned); the x-shifted pulls start at
+// an odd element and are assembled from the two aligned dwords around them.
+template <class S, class C, int BC, bool OBS>
+__global__ __launch_bounds__(256) void k_fused2(Lat L, Phys<C> P)
+{
+    static_assert(sizeof(S) == 2, "pairs of 16-bit elements");
+    const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * 2 + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i0 > L.l || j > L.m) return;
+    using R = typename CodecT<S, C>::raw;
+    const uint32_t bo = ((uint32_t)j * (uint32_t)L.rs + (uint32_t)i0) * 2u;
+    C x[2][9], o[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const int cx = k == 8 ? 0 : cx_of(k);
+        const char *base = pull_base<S>(L, k == 8 ? L.r : L.a[k], k);
+        uint32_t w;
+        if (cx == 0) {
+            w = *(const uint32_t *)(base + bo);
+        } else {
+            const uint32_t lo = *(const uint32_t *)(base + bo - 2), hi = *(const uint32_t *)(base + bo + 2);
+            w = (lo >> 16) | (hi << 16);
+        }
+        x[0][k] = CodecT<S, C>::dec(__builtin_bit_cast(R, (uint16_t)(w & 0xffffu)), P.inv);
+        x[1][k] = CodecT<S, C>::dec(__builtin_bit_cast(R, (uint16_t)(w >> 16)), P.inv);
+    }
+    uint32_t ow[9];
+    const bool two = i0 + 1 <= L.l;
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const int i = i0 + q;
+        fix_pulls<S, C, BC, OBS>(L, P, i, j, x[q]);
+        relax<C>(P, x[q], o);
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const uint32_t hb = __builtin_bit_cast(uint16_t, encode<S, C>(P, o[k], i, j, k));
+            ow[k] = q == 0 ? hb : (ow[k] | (hb << 16));
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        char *dst = (char *)pull_base<S>(L, k == 8 ? L.r : L.b[k], 8) + bo;
+        if (two)
+            __builtin_nontemporal_store(ow[k], (uint32_t *)dst);
+        else
+            __builtin_nontemporal_store((uint16_t)(ow[k] & 0xffffu), (uint16_t *)dst);
+    }
 }
 
 // ------------------------------------------------------------ diagnostics

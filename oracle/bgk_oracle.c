@@ -119,18 +119,10 @@ float orc_bf16_to_f32(uint16_t v)
 /* Stochastic rounding of half/bf16 stores: a hash of the value's low bits
  * and its position (i, j, k) is added below the kept bits before
  * truncation (the same rule as the HIP codec).                          */
-static uint32_t mix32(uint32_t x)
-{
-    x ^= x >> 16;
-    x *= 0x7feb352du;
-    x ^= x >> 15;
-    x *= 0x846ca68bu;
-    x ^= x >> 16;
-    return x;
-}
 static uint32_t salt_of(int i, int j, int k)
 {
-    return mix32((uint32_t)i * 0x9E3779B1u + (uint32_t)j * 0x85EBCA77u) + (uint32_t)k * 0x9E3779B9u;
+    return ((uint32_t)i & 0xffffffu) * 0x9E3779u + ((uint32_t)j & 0xffffffu) * 0x85EBCAu +
+           (uint32_t)k * 0x9E3779B9u;
 }
 static uint32_t sr_hash(uint32_t bits, uint32_t salt)
 {

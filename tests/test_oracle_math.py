@@ -316,7 +316,10 @@ def test_stochastic_rounding_brackets_and_is_unbiased():
         o.step(1)
         out[r] = o.raw()[:, 1:-1, 1:-1].view(np.float16).astype(np.float64)
     rn, sr = out[0], out[1]
-    ulp = np.abs(np.spacing(rn.astype(np.float16))).astype(np.float64)
+    r16 = rn.astype(np.float16)
+    up = np.nextafter(r16, np.float16(np.inf)).astype(np.float64) - rn
+    down = rn - np.nextafter(r16, np.float16(-np.inf)).astype(np.float64)
+    ulp = np.maximum(up, down)  # the larger step at binade boundaries
     assert np.all(np.abs(sr - rn) <= ulp * 1.0001)
     assert (sr != rn).mean() > 0.2  # the rule does change values
     dec_in = raw_in.view(np.float16).astype(np.float64)
