@@ -153,6 +153,17 @@ def copy_calib():
     print(json.dumps({"copy_calib_bytes_each_way": a.numel() * 4, "copies": 5}))
 
 
+def profiled_traffic():
+    """HBM bytes per launch of the FP32 kernel measured with FETCH_SIZE and
+    WRITE_SIZE (calibrated on a 1 GiB copy), from its committed summary"""
+    path = os.path.join(ROOT, "profiles", "k_fused_fp32_ldc4096_hbm_counters.md")
+    if os.path.exists(path):
+        m = re.search(r"traffic_bytes_per_launch:\s*([0-9]+)", open(path).read())
+        if m:
+            return int(m.group(1))
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +196,7 @@ def main():
     mlups = n * n * args.steps / r32["wall"] / 1e6
     kern_s = r32["kernel_ms"] / 1e3
     achieved = BYTES_PER_LUP["single"] * n * n / kern_s / 1e9
+    traffic = profiled_traffic()
 
     if not args.no_fp16:
         # FP16 shifted storage: Taylor-Green 8192^2 (tau = 0.8, u0 = 0.01) vs FP32
@@ -211,7 +223,12 @@ def main():
                                "Re=1e4 (svisc=0.04096, u0=0.1), boundary folded into the pull",
                    "global_batch": 1, "lattice": [n, n], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": (round(traffic / kern_s / 1e9, 1) if traffic else None),
+                     "algorithmic_bytes_per_launch": BYTES_PER_LUP["single"] * n * n,
+                     "traffic_bytes_per_launch": traffic,
+                     "kernel": "k_fused<float,float,DRIVEN> (one launch = one step)",
+                     "traffic_source": "profiles/k_fused_fp32_ldc4096_hbm_counters.md"},
         "kernel_ms_per_step": r32["kernel_ms"],
         "healthy": bool(r32["ok"]),
     }
