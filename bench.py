@@ -139,6 +139,29 @@ def precision_study(n=8192, steps=5000, amp=0.01, svisc=0.1, init_v=0):
     return res
 
 
+def serving_leg(n=48, seed=20261203, side=(128, 2048), steps=(1000, 10000), chunk=100,
+                budget=200e9):
+    """Mixed request stream (cavity / Poiseuille / Taylor-Green, sides and
+    Re log-uniform, steps uniform, FP32 or FP16 storage, Poisson arrivals at
+    200 req/s), served by continuous batching and by sequential FIFO solves
+    with the same kernels; p50/p99 time-to-solution and aggregate MLUPS."""
+    from bgk2d_gpu_amd import serve
+    reqs = serve.make_stream(n, seed=seed, side=side, steps=steps)
+    tiles = {p: serve.autotune(p, 1024) for p in ("single", "fp16")}
+    bo, bs = serve.serve_batched(reqs, budget_bytes=budget, chunk=chunk, check_every=1000,
+                                 tiles=tiles)
+    so, ss = serve.serve_sequential(reqs, check_every=1000)
+    b, q = serve.summarize(reqs, bo, bs), serve.summarize(reqs, so, ss)
+    same = all(bo[r.rid].status == so[r.rid].status for r in reqs)
+    diag_same = all(bo[r.rid].diag == so[r.rid].diag for r in reqs if bo[r.rid].status == "ok")
+    return {"stream": f"{n} requests, seed {seed}, sides log-uniform {side[0]}..{side[1]}, "
+                      f"Re log-uniform 100..1e4, steps uniform {steps[0]}..{steps[1]}, "
+                      "FP32/FP16 50/50, Poisson 200 req/s",
+            "batched": b, "sequential": q,
+            "aggregate_mlups_ratio": round(b["mlups"] / q["mlups"], 3),
+            "tiles": tiles, "same_outcomes": same, "same_diagnostics": diag_same}
+
+
 def copy_calib():
     """Plain device copies of a known size, for scaling the FETCH_SIZE /
     WRITE_SIZE counters to bytes."""
@@ -172,6 +195,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp16-steps", type=int, default=2000)
     ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--serve-n", type=int, default=48)
     ap.add_argument("--precision-study", action="store_true",
                     help="only run the FP32/FP16/BF16 vs FP64 Taylor-Green error study")
     ap.add_argument("--study-n", type=int, default=8192)
@@ -190,6 +216,9 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(each request or slab shards per GPU; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.serve_only:
+        print(json.dumps(serving_leg(args.serve_n)))
+        return
 
     n = 4096
     r32 = gpu_solve(n, "single", "driven", args.steps, args.warmup, 0.04096, 0.1)
@@ -240,6 +269,8 @@ def main():
                              "ratio": round(ml16 / ml32, 3),
                              "hbm_frac_fp16": round(36 * n3 * n3 / (t16["kernel_ms"] / 1e3) / 1e9
                                                     / HBM_PEAK_GBS, 4)}
+    if not args.no_serve:
+        out["serving"] = serving_leg(args.serve_n)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -7,6 +7,7 @@
 // bgk_kernels.hpp in the reference's phase order.
 #include "../../include/bgk2d.h"
 #include "bgk_kernels.hpp"
+#include "bgk_serve.hpp"
 
 #include <math.h>
 #include <stdarg.h>
@@ -426,6 +427,32 @@ int pack_t(bgk_ctx *c, int k, bool up)
         }                                                                                 \
     }()
 
+// pitched plane layout: a row holds nodes 0..l+1 from element off = A-1
+// (A elements = 128 bytes, so node 1 starts a line), plus two elements of
+// slack for the paired 16-bit accesses at the end of a row
+void layout_of(const bgk_params_t *p, int *esize, int *A, int *off, int *pitch, size_t *plane_elems)
+{
+    *esize = esize_of(p->prec);
+    *A = 128 / *esize;
+    *off = *A - 1;
+    *pitch = ((*A + p->lx + 3) + *A - 1) / *A * *A;
+    size_t pe = (size_t)(p->ly + 2) * (size_t)*pitch;
+    const size_t align = 256 / *esize;
+    *plane_elems = (pe + align - 1) / align * align;
+}
+
+// device bytes one context allocates (populations, mask, reduction scratch)
+size_t footprint_of(const bgk_params_t *p)
+{
+    int es, A, off, pitch;
+    size_t pe;
+    layout_of(p, &es, &A, &off, &pitch, &pe);
+    size_t obs_bytes = ((size_t)p->lx * p->ly + 255) / 256 * 256;
+    size_t rows_bytes = sizeof(double) * ROW_VALS * (size_t)p->ly;
+    size_t red_bytes = sizeof(double) * 3 * (size_t)(p->lx > p->ly ? p->lx : p->ly) + 64;
+    return pe * (size_t)es * 17 + obs_bytes + rows_bytes + red_bytes;
+}
+
 int check_ctx(bgk_ctx *c)
 {
     if (!c) return BGK_E_INVALID;
@@ -476,15 +503,7 @@ int bgk_create(const bgk_params_t *p, bgk_ctx_t **out)
             return BGK_E_INVALID;  // power of two only: scaling must be exact
         }
     }
-    c->esize = esize_of(p->prec);
-    c->A = 128 / c->esize;
-    c->off = c->A - 1;
-    // a row holds nodes 0..l+1 from element off = A-1, plus two elements of
-    // slack for the paired 16-bit accesses at the end of a row
-    c->pitch = ((c->A + c->l + 3) + c->A - 1) / c->A * c->A;
-    size_t pe = (size_t)(c->m + 2) * (size_t)c->pitch;
-    const size_t align = 256 / c->esize;
-    c->plane_elems = (pe + align - 1) / align * align;
+    layout_of(p, &c->esize, &c->A, &c->off, &c->pitch, &c->plane_elems);
     c->imin = c->l;
     c->jmin = c->m;
     c->imax = 0;
@@ -838,6 +857,316 @@ int bgk_upload(bgk_ctx_t *c, int32_t k, const void *host)
 }
 
 void *bgk_stream(bgk_ctx_t *c) { return c ? (void *)c->s : nullptr; }
+
+// ===================================================================== batch
+
+struct bgk_batch {
+    int dev;
+    hipStream_t s;
+    size_t budget, used;
+    struct Slot {
+        bgk_ctx *ctx;
+        long long remaining, done;
+        size_t bytes;
+        int tw_log2, th;
+    };
+    std::vector<Slot> slots;
+    bgk::Member *d_members, *h_members;
+    int2 *d_tiles, *h_tiles;
+    int *d_ntiles, *h_ntiles;
+    size_t tiles_cap;
+    std::map<int, hipGraphExec_t> graphs;
+    int grid;
+    hipEvent_t ev0, ev1;
+    double last_ms;
+    std::string err;
+};
+
+namespace {
+
+int bfail(bgk_batch_t *b, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (b) b->err = buf;
+    return code;
+}
+
+#define BHIPCHK(b, expr)                                                                           \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) return bfail((b), BGK_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// tile codes: 1 = 64x4, 2 = 128x2, 3 = 256x1, 4 = 32x8
+bool tile_shape(int code, int *tw_log2, int *th)
+{
+    switch (code) {
+    case 0:
+    case 1: *tw_log2 = 6; *th = 4; return true;
+    case 2: *tw_log2 = 7; *th = 2; return true;
+    case 3: *tw_log2 = 8; *th = 1; return true;
+    case 4: *tw_log2 = 5; *th = 8; return true;
+    default: return false;
+    }
+}
+
+void drop_graphs(bgk_batch_t *b)
+{
+    for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);
+    b->graphs.clear();
+}
+
+int ensure_tiles(bgk_batch_t *b, size_t n)
+{
+    if (n <= b->tiles_cap) return BGK_OK;
+    size_t cap = b->tiles_cap ? b->tiles_cap : 4096;
+    while (cap < n) cap *= 2;
+    if (b->d_tiles) (void)hipFree(b->d_tiles);
+    if (b->h_tiles) (void)hipHostFree(b->h_tiles);
+    b->d_tiles = nullptr;
+    b->h_tiles = nullptr;
+    b->tiles_cap = 0;
+    BHIPCHK(b, hipMalloc(&b->d_tiles, sizeof(int2) * cap));
+    BHIPCHK(b, hipHostMalloc(&b->h_tiles, sizeof(int2) * cap));
+    b->tiles_cap = cap;
+    drop_graphs(b);  // the graphs hold the old table address
+    return BGK_OK;
+}
+
+}  // namespace
+
+int bgk_batch_create(int32_t device, double budget_bytes, int32_t max_members, bgk_batch_t **out)
+{
+    if (!out || max_members < 1 || max_members > 65536 || !(budget_bytes > 0)) return BGK_E_INVALID;
+    *out = nullptr;
+    if (hipSetDevice(device) != hipSuccess) return BGK_E_HIP;
+    bgk_batch_t *b = new bgk_batch_t();
+    b->dev = device;
+    b->budget = (size_t)budget_bytes;
+    b->used = 0;
+    b->slots.assign((size_t)max_members, bgk_batch_t::Slot{nullptr, 0, 0, 0, 6, 4});
+    b->tiles_cap = 0;
+    b->d_tiles = nullptr;
+    b->h_tiles = nullptr;
+    b->last_ms = 0.0;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
+        cus = 256;
+    b->grid = cus * 8;
+    if (hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return BGK_E_HIP;
+    }
+    if (hipMalloc(&b->d_members, sizeof(bgk::Member) * max_members) != hipSuccess ||
+        hipHostMalloc(&b->h_members, sizeof(bgk::Member) * max_members) != hipSuccess ||
+        hipMalloc(&b->d_ntiles, sizeof(int)) != hipSuccess || hipHostMalloc(&b->h_ntiles, sizeof(int)) != hipSuccess) {
+        (void)hipStreamDestroy(b->s);
+        delete b;
+        return BGK_E_NOMEM;
+    }
+    (void)hipEventCreate(&b->ev0);
+    (void)hipEventCreate(&b->ev1);
+    *out = b;
+    return BGK_OK;
+}
+
+int bgk_batch_destroy(bgk_batch_t *b)
+{
+    if (!b) return BGK_E_INVALID;
+    (void)hipSetDevice(b->dev);
+    (void)hipStreamSynchronize(b->s);
+    drop_graphs(b);
+    for (auto &sl : b->slots)
+        if (sl.ctx) bgk_destroy(sl.ctx);
+    (void)hipFree(b->d_members);
+    (void)hipHostFree(b->h_members);
+    if (b->d_tiles) (void)hipFree(b->d_tiles);
+    if (b->h_tiles) (void)hipHostFree(b->h_tiles);
+    (void)hipFree(b->d_ntiles);
+    (void)hipHostFree(b->h_ntiles);
+    (void)hipEventDestroy(b->ev0);
+    (void)hipEventDestroy(b->ev1);
+    (void)hipStreamDestroy(b->s);
+    delete b;
+    return BGK_OK;
+}
+
+const char *bgk_batch_last_error(const bgk_batch_t *b) { return b ? b->err.c_str() : "null batch"; }
+
+int bgk_footprint(const bgk_params_t *p, double *bytes)
+{
+    if (!p || !bytes || p->lx < 2 || p->ly < 2 || p->prec < 0 || p->prec > 4) return BGK_E_INVALID;
+    *bytes = (double)footprint_of(p);
+    return BGK_OK;
+}
+
+int bgk_batch_add(bgk_batch_t *b, const bgk_params_t *p, int32_t init_v, double amplitude, int64_t nsteps,
+                  int32_t tile, int32_t *slot)
+{
+    if (!b || !p || !slot || nsteps < 0) return BGK_E_INVALID;
+    *slot = -1;
+    if (p->impl != BGK_IMPL_FUSED || p->bc == BGK_BC_STEP || p->obstacle ||
+        !(p->prec == BGK_PREC_SINGLE || p->prec == BGK_PREC_FP16 || p->prec == BGK_PREC_BF16))
+        return bfail(b, BGK_E_INVALID, "batch members are FUSED, without obstacle or step, fp32/fp16/bf16");
+    if (p->device != b->dev) return bfail(b, BGK_E_INVALID, "member on device %d, batch on %d", p->device, b->dev);
+    int tw_log2, th;
+    if (!tile_shape(tile, &tw_log2, &th)) return bfail(b, BGK_E_INVALID, "unknown tile code %d", tile);
+    const size_t bytes = footprint_of(p);
+    if (b->used + bytes > b->budget) return bfail(b, BGK_E_NOMEM, "HBM budget: %zu + %zu > %zu", b->used, bytes, b->budget);
+    int free_slot = -1;
+    for (size_t q = 0; q < b->slots.size(); q++)
+        if (!b->slots[q].ctx) {
+            free_slot = (int)q;
+            break;
+        }
+    if (free_slot < 0) return bfail(b, BGK_E_NOMEM, "all %zu slots in use", b->slots.size());
+    bgk_ctx_t *c = nullptr;
+    int r = bgk_create(p, &c);
+    if (r != BGK_OK) return bfail(b, r, "bgk_create failed (%d)", r);
+    r = bgk_init(c, init_v, amplitude);
+    if (r != BGK_OK) {
+        bfail(b, r, "bgk_init: %s", c->err.c_str());
+        bgk_destroy(c);
+        return r;
+    }
+    b->slots[free_slot] = bgk_batch_t::Slot{c, (long long)nsteps, 0, bytes, tw_log2, th};
+    b->used += bytes;
+    *slot = free_slot;
+    return BGK_OK;
+}
+
+int bgk_batch_ctx(bgk_batch_t *b, int32_t slot, bgk_ctx_t **ctx)
+{
+    if (!b || !ctx || slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].ctx) return BGK_E_INVALID;
+    *ctx = b->slots[slot].ctx;
+    return BGK_OK;
+}
+
+int bgk_batch_remaining(bgk_batch_t *b, int32_t slot, int64_t *remaining)
+{
+    if (!b || !remaining || slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].ctx) return BGK_E_INVALID;
+    *remaining = b->slots[slot].remaining;
+    return BGK_OK;
+}
+
+int bgk_batch_remove(bgk_batch_t *b, int32_t slot)
+{
+    if (!b || slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].ctx) return BGK_E_INVALID;
+    (void)hipStreamSynchronize(b->s);
+    bgk_destroy(b->slots[slot].ctx);
+    b->used -= b->slots[slot].bytes;
+    b->slots[slot] = bgk_batch_t::Slot{nullptr, 0, 0, 0, 6, 4};
+    return BGK_OK;
+}
+
+int bgk_batch_used(const bgk_batch_t *b, double *bytes)
+{
+    if (!b || !bytes) return BGK_E_INVALID;
+    *bytes = (double)b->used;
+    return BGK_OK;
+}
+
+// Advance every member with steps left by up to `chunk` steps: one graph of
+// `chunk` k_batch launches.  *finished = members that reached zero.
+int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
+{
+    if (!b || chunk < 1 || chunk > 4096) return BGK_E_INVALID;
+    BHIPCHK(b, hipSetDevice(b->dev));
+    int nm = 0;
+    size_t nt = 0;
+    std::vector<int> members_slot;
+    for (size_t q = 0; q < b->slots.size(); q++) {
+        const auto &sl = b->slots[q];
+        if (!sl.ctx || sl.remaining <= 0) continue;
+        const bgk_ctx *c = sl.ctx;
+        const int tw = 1 << sl.tw_log2;
+        nt += (size_t)((c->l + tw - 1) / tw) * (size_t)((c->m + sl.th - 1) / sl.th);
+        members_slot.push_back((int)q);
+        nm++;
+    }
+    if (finished) *finished = 0;
+    if (nm == 0) {
+        b->last_ms = 0.0;
+        return BGK_OK;
+    }
+    int r = ensure_tiles(b, nt);
+    if (r) return r;
+    size_t t = 0;
+    for (int mi = 0; mi < nm; mi++) {
+        const auto &sl = b->slots[members_slot[mi]];
+        const bgk_ctx *c = sl.ctx;
+        bgk::Member &d = b->h_members[mi];
+        for (int k = 0; k < 8; k++) {
+            d.set[0][k] = c->plane[k];
+            d.set[1][k] = c->plane[8 + k];
+        }
+        d.r = c->plane[16];
+        d.l = c->l;
+        d.m = c->m;
+        d.off = c->off;
+        d.rs = c->pitch;
+        d.store = c->p.prec == BGK_PREC_FP16 ? bgk::MK_F16 : (c->p.prec == BGK_PREC_BF16 ? bgk::MK_BF16 : bgk::MK_F32);
+        d.bc = c->p.bc;  // BGK_BC_* and BcKind share the numbering of the foldable variants
+        const int tw = 1 << sl.tw_log2;
+        d.tiles_x = (c->l + tw - 1) / tw;
+        d.tw_log2 = sl.tw_log2;
+        d.th = sl.th;
+        d.parity0 = c->cur;
+        d.remaining = (int)(sl.remaining < chunk ? sl.remaining : chunk);
+        d.P = phys<float>(c);
+        const int ntm = d.tiles_x * ((c->m + sl.th - 1) / sl.th);
+        for (int q = 0; q < ntm; q++) b->h_tiles[t++] = make_int2(mi, q);
+    }
+    *b->h_ntiles = (int)nt;
+    BHIPCHK(b, hipMemcpyAsync(b->d_members, b->h_members, sizeof(bgk::Member) * nm, hipMemcpyHostToDevice, b->s));
+    BHIPCHK(b, hipMemcpyAsync(b->d_tiles, b->h_tiles, sizeof(int2) * nt, hipMemcpyHostToDevice, b->s));
+    BHIPCHK(b, hipMemcpyAsync(b->d_ntiles, b->h_ntiles, sizeof(int), hipMemcpyHostToDevice, b->s));
+    hipGraphExec_t g;
+    auto it = b->graphs.find(chunk);
+    if (it == b->graphs.end()) {
+        BHIPCHK(b, hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+        for (int tt = 0; tt < chunk; tt++)
+            hipLaunchKernelGGL(bgk::k_batch, dim3(b->grid), dim3(256), 0, b->s, (const bgk::Member *)b->d_members,
+                               (const int2 *)b->d_tiles, (const int *)b->d_ntiles, tt);
+        hipGraph_t gr = nullptr;
+        BHIPCHK(b, hipStreamEndCapture(b->s, &gr));
+        hipError_t e = hipGraphInstantiate(&g, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (e != hipSuccess) return bfail(b, BGK_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+        b->graphs[chunk] = g;
+    } else {
+        g = it->second;
+    }
+    BHIPCHK(b, hipEventRecord(b->ev0, b->s));
+    BHIPCHK(b, hipGraphLaunch(g, b->s));
+    BHIPCHK(b, hipEventRecord(b->ev1, b->s));
+    BHIPCHK(b, hipEventSynchronize(b->ev1));
+    float ms = 0.f;
+    BHIPCHK(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    int nfin = 0;
+    for (int mi = 0; mi < nm; mi++) {
+        auto &sl = b->slots[members_slot[mi]];
+        const int steps = b->h_members[mi].remaining;
+        sl.ctx->cur = (sl.ctx->cur + steps) & 1;
+        sl.remaining -= steps;
+        sl.done += steps;
+        if (sl.remaining == 0) nfin++;
+    }
+    if (finished) *finished = nfin;
+    return BGK_OK;
+}
+
+int bgk_batch_last_ms(const bgk_batch_t *b, double *ms)
+{
+    if (!b || !ms) return BGK_E_INVALID;
+    *ms = b->last_ms;
+    return BGK_OK;
+}
 
 int bgk_last_run_ms(const bgk_ctx_t *c, double *ms)
 {

@@ -439,12 +439,11 @@ __device__ __forceinline__ const char *pull_base(const Lat &L, const void *plane
     return (const char *)plane + ((long long)L.off - (long long)cy_of(k) * L.rs - cx_of(k)) * (long long)sizeof(S);
 }
 
+// one FUSED step of interior node (i, j): pull (boundary folded), relax,
+// store into the other set (and the rest population in place)
 template <class S, class C, int BC, bool OBS>
-__global__ __launch_bounds__(256) void k_fused(Lat L, Phys<C> P)
+__device__ __forceinline__ void fused_node(const Lat &L, const Phys<C> &P, int i, int j)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    if (i > L.l || j > L.m) return;
     using R = typename CodecT<S, C>::raw;
     const uint32_t bo = ((uint32_t)j * (uint32_t)L.rs + (uint32_t)i) * (uint32_t)sizeof(S);
     C x[9], o[9];
@@ -458,6 +457,15 @@ __global__ __launch_bounds__(256) void k_fused(Lat L, Phys<C> P)
 #pragma unroll
     for (int k = 0; k < 8; k++) st_nt((R *)(pull_base<S>(L, L.b[k], 8) + bo), encode<S, C>(P, o[k], i, j, k));
     st_nt((R *)(pull_base<S>(L, L.r, 8) + bo), encode<S, C>(P, o[8], i, j, 8));
+}
+
+template <class S, class C, int BC, bool OBS>
+__global__ __launch_bounds__(256) void k_fused(Lat L, Phys<C> P)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > L.l || j > L.m) return;
+    fused_node<S, C, BC, OBS>(L, P, i, j);
 }
 
 // The same step for 16-bit storage, two nodes (i0, i0+1) per thread with

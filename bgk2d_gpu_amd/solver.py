@@ -24,7 +24,10 @@ EXPORTS = ("bgk_create", "bgk_destroy", "bgk_last_error", "bgk_version", "bgk_co
            "bgk_build_obs", "bgk_set_obstacle", "bgk_get_obstacle", "bgk_init",
            "bgk_boundaries", "bgk_propagation", "bgk_collision", "bgk_run", "bgk_sync",
            "bgk_diagno", "bgk_varm", "bgk_prof_i", "bgk_prof_j", "bgk_probe", "bgk_check",
-           "bgk_download", "bgk_upload", "bgk_storage_bytes", "bgk_stream", "bgk_last_run_ms")
+           "bgk_download", "bgk_upload", "bgk_storage_bytes", "bgk_stream", "bgk_last_run_ms",
+           "bgk_batch_create", "bgk_batch_destroy", "bgk_batch_last_error", "bgk_footprint",
+           "bgk_batch_add", "bgk_batch_ctx", "bgk_batch_remaining", "bgk_batch_remove",
+           "bgk_batch_used", "bgk_batch_run", "bgk_batch_last_ms")
 
 
 class BgkParams(C.Structure):

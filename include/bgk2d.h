@@ -160,6 +160,36 @@ int bgk_storage_bytes(const bgk_ctx_t *ctx);
 void *bgk_stream(bgk_ctx_t *ctx);
 int bgk_last_run_ms(const bgk_ctx_t *ctx, double *ms);
 
+/* ------------------------------------------------------------------------
+ * Batched solving (no counterpart in the reference, which runs one solve
+ * per process): many independent FUSED solves of any size, boundary
+ * (driven, periodic, channel, inflow, couette; no obstacle) and storage kind
+ * (fp32, fp16, bf16) stay resident in HBM and advance together, one
+ * persistent-kernel launch per step for all of them, chunks of steps
+ * replayed from HIP graphs.  A member's results are bit-identical to the
+ * same solve run alone with bgk_run.  Members are admitted while their
+ * device footprint fits the byte budget given at creation.               */
+typedef struct bgk_batch bgk_batch_t;
+
+int bgk_batch_create(int32_t device, double budget_bytes, int32_t max_members, bgk_batch_t **out);
+int bgk_batch_destroy(bgk_batch_t *b);
+const char *bgk_batch_last_error(const bgk_batch_t *b);
+/* device bytes a context with these parameters allocates */
+int bgk_footprint(const bgk_params_t *p, double *bytes);
+/* create + init a member that will run nsteps steps; tile: 0 default,
+ * 1 = 64x4, 2 = 128x2, 3 = 256x1, 4 = 32x8 nodes per block.  Returns
+ * BGK_E_NOMEM (nothing allocated) when the budget or the slots are full. */
+int bgk_batch_add(bgk_batch_t *b, const bgk_params_t *p, int32_t init_v, double amplitude, int64_t nsteps,
+                  int32_t tile, int32_t *slot);
+/* the member's context, for bgk_diagno/bgk_check/bgk_download...        */
+int bgk_batch_ctx(bgk_batch_t *b, int32_t slot, bgk_ctx_t **ctx);
+int bgk_batch_remaining(bgk_batch_t *b, int32_t slot, int64_t *remaining);
+int bgk_batch_remove(bgk_batch_t *b, int32_t slot);
+int bgk_batch_used(const bgk_batch_t *b, double *bytes);
+/* advance every member with steps left by min(chunk, remaining) steps   */
+int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished);
+int bgk_batch_last_ms(const bgk_batch_t *b, double *ms);
+
 #ifdef __cplusplus
 }
 #endif
