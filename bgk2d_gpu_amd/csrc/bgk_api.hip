@@ -1,6 +1,6 @@
 // bgk_api.hip -- C ABI and runtime of libbgk2d_hip.so (see include/bgk2d.h).
 //
-// A context owns one HBM allocation (17 pitched population planes plus
+// A context owns one HBM allocation (18 pitched population planes plus
 // the obstacle mask and reduction scratch), one HIP stream and the HIP
 // graphs it replays.  Host code derives the reference's constants in the
 // compute kind (hencol, build_bcond) and launches the kernels of
@@ -28,8 +28,10 @@ struct bgk_ctx {
     int dev;
     hipStream_t s;
     void *mem;
-    void *plane[17];  // [0..7] set 0, [8..15] set 1, [16] pop 19
+    void *plane[18];  // [0..7] set 0, [8..15] set 1, [16], [17] pop 19
     int cur;          // which set is `a`
+    int rcur;         // which pop-19 plane is current (16 + rcur)
+    int tb_h;         // strip height of the two-step kernel (0: not sized yet)
     uint8_t *d_obs;
     int imin, imax, jmin, jmax, nobs;
     std::vector<int32_t> h_obs;
@@ -133,7 +135,8 @@ Lat lat(const bgk_ctx *c)
         L.a[k] = c->plane[a * 8 + k];
         L.b[k] = c->plane[b * 8 + k];
     }
-    L.r = c->plane[16];
+    L.r = c->plane[16 + c->rcur];
+    L.rn = c->plane[17 - c->rcur];
     L.obs = c->d_obs;
     L.l = c->l;
     L.m = c->m;
@@ -270,27 +273,122 @@ int launch_step(bgk_ctx *c)
     return r;
 }
 
-// Steps are replayed from HIP graphs of GRAPH_STEPS launches each (an even
-// count, so the FUSED set parity is the same before and after a replay);
-// one graph per starting parity, built on first use.
+// Two FUSED steps per pass over HBM (k_fused_tb): the default for fp32 and
+// fp64 storage, where it beats one step per pass; for FP16/BF16 storage
+// the paired one-step kernel is as fast, so it stays the default there.
+bool use_tb(const bgk_ctx *c)
+{
+    if (!foldable(c)) return false;
+    if (c->p.tblock == 2) return true;
+    return c->p.tblock == 0 && c->esize >= 4;
+}
+
+// Strip height of k_fused_tb: as many strips of rows as make the grid one
+// round of the blocks the device holds at once, so no CU idles in a
+// partial last round.
+template <class S, class C, int BC, bool OBS>
+int size_tb(bgk_ctx *c)
+{
+    int per_cu = 0, cus = 0;
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused_tb<S, C, BC, OBS>, 256, 0));
+    HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->dev));
+    if (per_cu < 1) per_cu = 1;
+    if (cus < 1) cus = 1;
+    const long long slots = (long long)per_cu * cus;
+    const long long strips = (c->l + TB_TW - 1) / TB_TW;
+    long long per_strip = slots / strips;
+    if (per_strip < 1) per_strip = 1;
+    c->tb_h = (int)((c->m + per_strip - 1) / per_strip);
+    return BGK_OK;
+}
+
+template <class S, class C, bool OBS>
+int size_tb_o(bgk_ctx *c)
+{
+    switch (c->p.bc) {
+    case BGK_BC_PERIODIC: return size_tb<S, C, BK_PERIODIC, OBS>(c);
+    case BGK_BC_CHANNEL: return size_tb<S, C, BK_CHANNEL, OBS>(c);
+    case BGK_BC_INFLOW: return size_tb<S, C, BK_INFLOW, OBS>(c);
+    case BGK_BC_COUETTE: return size_tb<S, C, BK_COUETTE, OBS>(c);
+    default: return size_tb<S, C, BK_DRIVEN, OBS>(c);
+    }
+}
+
+template <class S, class C, int BC, bool OBS>
+int launch_tb_t(bgk_ctx *c)
+{
+    if (c->tb_h == 0) {
+        int r = size_tb<S, C, BC, OBS>(c);
+        if (r != BGK_OK) return r;
+    }
+    Lat L = lat(c);
+    Phys<C> P = phys<C>(c);
+    const dim3 g((c->l + TB_TW - 1) / TB_TW, (c->m + c->tb_h - 1) / c->tb_h);
+    hipLaunchKernelGGL((k_fused_tb<S, C, BC, OBS>), g, dim3(256), 0, c->s, L, P, c->tb_h);
+    c->cur = 1 - c->cur;
+    c->rcur = 1 - c->rcur;
+    HIPCHK(c, hipGetLastError());
+    return BGK_OK;
+}
+
+template <class S, class C, bool OBS>
+int launch_tb_o(bgk_ctx *c)
+{
+    switch (c->p.bc) {
+    case BGK_BC_PERIODIC: return launch_tb_t<S, C, BK_PERIODIC, OBS>(c);
+    case BGK_BC_CHANNEL: return launch_tb_t<S, C, BK_CHANNEL, OBS>(c);
+    case BGK_BC_INFLOW: return launch_tb_t<S, C, BK_INFLOW, OBS>(c);
+    case BGK_BC_COUETTE: return launch_tb_t<S, C, BK_COUETTE, OBS>(c);
+    default: return launch_tb_t<S, C, BK_DRIVEN, OBS>(c);
+    }
+}
+
+// n complete steps: pairs through k_fused_tb where it is used, the rest one
+// step at a time
+template <class S, class C>
+int launch_steps(bgk_ctx *c, int n)
+{
+    int t = 0;
+    if (use_tb(c)) {
+        const bool obs = c->p.obstacle && c->nobs > 0;
+        for (; t + 2 <= n; t += 2) {
+            int r = obs ? launch_tb_o<S, C, true>(c) : launch_tb_o<S, C, false>(c);
+            if (r != BGK_OK) return r;
+        }
+    }
+    for (; t < n; t++) {
+        int r = launch_step<S, C>(c);
+        if (r != BGK_OK) return r;
+    }
+    return BGK_OK;
+}
+
+// Steps are replayed from HIP graphs of GRAPH_STEPS steps each (an even
+// count, so the FUSED set parities are the same before and after a
+// replay); one graph per starting parity, built on first use.
 constexpr int GRAPH_STEPS = 32;
 
 template <class S, class C>
 int graph_for(bgk_ctx *c, hipGraphExec_t *out)
 {
-    const long long key = c->cur;
+    const long long key = c->cur + 2 * c->rcur;
     auto it = c->graphs.find(key);
     if (it != c->graphs.end()) {
         *out = it->second;
         return BGK_OK;
     }
-    const int cur0 = c->cur;
+    if (use_tb(c) && c->tb_h == 0) {
+        // size the two-step kernel before the capture
+        int r = c->p.obstacle && c->nobs > 0 ? size_tb_o<S, C, true>(c) : size_tb_o<S, C, false>(c);
+        if (r != BGK_OK) return r;
+    }
+    const int cur0 = c->cur, rcur0 = c->rcur;
     HIPCHK(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
-    int r = BGK_OK;
-    for (int t = 0; t < GRAPH_STEPS && r == BGK_OK; t++) r = launch_step<S, C>(c);
+    int r = launch_steps<S, C>(c, GRAPH_STEPS);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->s, &g);
     c->cur = cur0;
+    c->rcur = rcur0;
     if (r != BGK_OK) {
         if (g) (void)hipGraphDestroy(g);
         return r;
@@ -316,10 +414,8 @@ int run_t(bgk_ctx *c, int nsteps)
         if (r != BGK_OK) return r;
         for (; t + GRAPH_STEPS <= nsteps; t += GRAPH_STEPS) HIPCHK(c, hipGraphLaunch(g, c->s));
     }
-    for (; t < nsteps; t++) {
-        int r = launch_step<S, C>(c);
-        if (r != BGK_OK) return r;
-    }
+    int r = launch_steps<S, C>(c, nsteps - t);
+    if (r != BGK_OK) return r;
     HIPCHK(c, hipEventRecord(c->ev1, c->s));
     HIPCHK(c, hipEventSynchronize(c->ev1));
     float ms = 0.f;
@@ -395,7 +491,7 @@ int line_t(bgk_ctx *c, int mode, int c0, int c1, int n)
 template <class S>
 int fill_t(bgk_ctx *c, S v)
 {
-    long long n = (long long)c->plane_elems * 17;
+    long long n = (long long)c->plane_elems * 18;
     hipLaunchKernelGGL((k_fill<S>), dim3(2048), dim3(256), 0, c->s, c->mem, n, v);
     HIPCHK(c, hipGetLastError());
     return BGK_OK;
@@ -405,7 +501,7 @@ template <class S>
 int pack_t(bgk_ctx *c, int k, bool up)
 {
     Lat L = lat(c);
-    void *plane = k == 8 ? c->plane[16] : c->plane[c->cur * 8 + k];
+    void *plane = k == 8 ? c->plane[16 + c->rcur] : c->plane[c->cur * 8 + k];
     if (up)
         hipLaunchKernelGGL((k_unpack<S>), grid2(c->l + 2, c->m + 2, B2), B2, 0, c->s, L, plane, (const S *)c->d_stage);
     else
@@ -450,7 +546,7 @@ size_t footprint_of(const bgk_params_t *p)
     size_t obs_bytes = ((size_t)p->lx * p->ly + 255) / 256 * 256;
     size_t rows_bytes = sizeof(double) * ROW_VALS * (size_t)p->ly;
     size_t red_bytes = sizeof(double) * 3 * (size_t)(p->lx > p->ly ? p->lx : p->ly) + 64;
-    return pe * (size_t)es * 17 + obs_bytes + rows_bytes + red_bytes;
+    return pe * (size_t)es * 18 + obs_bytes + rows_bytes + red_bytes;
 }
 
 int check_ctx(bgk_ctx *c)
@@ -488,6 +584,7 @@ int bgk_create(const bgk_params_t *p, bgk_ctx_t **out)
     if (!(p->svisc > 0.0)) return BGK_E_INVALID;
     if (p->form != 0 && p->form != 1) return BGK_E_INVALID;
     if (p->rounding != 0 && p->rounding != 1) return BGK_E_INVALID;
+    if (p->tblock < 0 || p->tblock > 2) return BGK_E_INVALID;
     bgk_ctx *c = new bgk_ctx();
     c->p = *p;
     c->l = p->lx;
@@ -510,6 +607,8 @@ int bgk_create(const bgk_params_t *p, bgk_ctx_t **out)
     c->jmax = 0;
     c->nobs = 0;
     c->cur = 0;
+    c->rcur = 0;
+    c->tb_h = 0;
     c->last_ms = 0.0;
     c->inited = false;
     c->stage_bytes = 0;
@@ -524,7 +623,7 @@ int bgk_create(const bgk_params_t *p, bgk_ctx_t **out)
     }
     hipEventCreate(&c->ev0);
     hipEventCreate(&c->ev1);
-    size_t bytes = c->plane_elems * (size_t)c->esize * 17;
+    size_t bytes = c->plane_elems * (size_t)c->esize * 18;
     size_t obs_bytes = ((size_t)c->l * c->m + 255) / 256 * 256;
     size_t rows_bytes = sizeof(double) * ROW_VALS * (size_t)c->m;
     size_t red_bytes = sizeof(double) * 3 * (size_t)(c->l > c->m ? c->l : c->m) + 64;
@@ -540,7 +639,7 @@ int bgk_create(const bgk_params_t *p, bgk_ctx_t **out)
         delete c;
         return BGK_E_NOMEM;
     }
-    for (int q = 0; q < 17; q++) c->plane[q] = (char *)c->mem + (size_t)q * c->plane_elems * c->esize;
+    for (int q = 0; q < 18; q++) c->plane[q] = (char *)c->mem + (size_t)q * c->plane_elems * c->esize;
     hipMemsetAsync(c->d_obs, 0, obs_bytes, c->s);
     c->h_obs.assign((size_t)c->l * c->m, 0);
     // alloca: huge(mykind) = 2147483647 converted to the storage kind
@@ -1104,7 +1203,7 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
             d.set[0][k] = c->plane[k];
             d.set[1][k] = c->plane[8 + k];
         }
-        d.r = c->plane[16];
+        d.r = c->plane[16 + c->rcur];
         d.l = c->l;
         d.m = c->m;
         d.off = c->off;

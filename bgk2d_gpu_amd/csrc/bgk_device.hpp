@@ -132,7 +132,8 @@ struct Codec<bf16_t, float> {
 struct Lat {
     void *a[8];  // current set, pops 01..17
     void *b[8];  // other set (FUSED target / ORIGINAL streamed copy)
-    void *r;     // pop 19
+    void *r;     // pop 19 (current)
+    void *rn;    // second pop-19 plane: target of the two-step kernel
     const uint8_t *obs;  // l*m solid mask (x fastest) or null
     int l, m;
     int pitch, off;

@@ -356,26 +356,46 @@ __global__ void k_colmc(Lat L, Phys<C> P)
 //             column l+1 -> equilibrium of the momentum of (l, row).
 enum BcKind { BK_DRIVEN = 0, BK_PERIODIC = 1, BK_CHANNEL = 2, BK_INFLOW = 3, BK_COUETTE = 4 };
 
+// Periodic directions of each foldable variant
+template <int BC>
+struct Per {
+    static constexpr bool x = BC == BK_PERIODIC || BC == BK_CHANNEL || BC == BK_COUETTE;
+    static constexpr bool y = BC == BK_PERIODIC || BC == BK_INFLOW;
+};
+
+// Post-collision value of population k at node (si, sj) of the current set,
+// read from HBM; coordinates one step outside the interior along a
+// periodic direction wrap to the opposite edge.
 template <class S, class C, int BC>
-__device__ C folded(const Lat &L, const Phys<C> &P, int k, int i, int j, int si, int sj)
-{
-    const int l = L.l, m = L.m;
-    if (BC == BK_PERIODIC) {
-        si = si < 1 ? l : (si > l ? 1 : si);
-        sj = sj < 1 ? m : (sj > m ? 1 : sj);
+struct HbmAcc {
+    const Lat &L;
+    const Phys<C> &P;
+    __device__ __forceinline__ C operator()(int k, int si, int sj) const
+    {
+        if (Per<BC>::x) si = si < 1 ? si + L.l : (si > L.l ? si - L.l : si);
+        if (Per<BC>::y) sj = sj < 1 ? sj + L.m : (sj > L.m ? sj - L.m : sj);
         return get<S, C>(L, P, L.a, k, si, sj);
     }
+};
+
+// Value that the boundary pass would have stored at halo node (si, sj) for
+// population k, as read by interior node (i, j); `a` returns current
+// post-collision interior values (and wraps periodic directions).
+template <class S, class C, int BC, class A>
+__device__ C folded_a(const A &a, const Phys<C> &P, int l, int m, int k, int i, int j, int si, int sj)
+{
+    if (BC == BK_PERIODIC) return a(k, si, sj);
     if (BC == BK_INFLOW) {
-        sj = sj < 1 ? m : (sj > m ? 1 : sj);
-        if (si < 1) return rt<S, C>(P, P.inflow[k], 0, sj, k);  // pops 01,03,05 from x = 0
+        const int sjw = sj < 1 ? m : (sj > m ? 1 : sj);  // the y pass runs after the x pass
+        if (si < 1) return rt<S, C>(P, P.inflow[k], 0, sjw, k);  // pops 01,03,05 from x = 0
         if (si > l) {
             // outflow equilibrium from the momentum of (l, sj)
             const C rf = K<C>::rf, qf = K<C>::qf, p1 = K<C>::p1, p2 = K<C>::p2;
             const C rhoinv = (C)1.0, crho = (C)1.0, cte1 = (C)0.0;
-            C x01 = get<S, C>(L, P, L.a, 0, l, sj), x03 = get<S, C>(L, P, L.a, 1, l, sj);
-            C x05 = get<S, C>(L, P, L.a, 2, l, sj), x08 = get<S, C>(L, P, L.a, 3, l, sj);
-            C x10 = get<S, C>(L, P, L.a, 4, l, sj), x12 = get<S, C>(L, P, L.a, 5, l, sj);
-            C x14 = get<S, C>(L, P, L.a, 6, l, sj), x17 = get<S, C>(L, P, L.a, 7, l, sj);
+            C x01 = a(0, l, sj), x03 = a(1, l, sj);
+            C x05 = a(2, l, sj), x08 = a(3, l, sj);
+            C x10 = a(4, l, sj), x12 = a(5, l, sj);
+            C x14 = a(6, l, sj), x17 = a(7, l, sj);
             C xj = ((x03 - x12) + (x01 - x10) + (x05 - x14)) * rhoinv;
             C yj = ((x03 - x01) + (x12 - x10) + (x08 - x17)) * rhoinv;
             C cvsq = xj * xj + yj * yj;
@@ -386,35 +406,44 @@ __device__ C folded(const Lat &L, const Phys<C> &P, int k, int i, int j, int si,
                 v = crho * p2 * (cte1 + (rf * (-xj + yj) + qf * ((C)3.0 * (xj - yj) * (xj - yj) - cvsq)));
             else
                 v = crho * p1 * (cte1 + (rf * (-xj) + qf * ((C)3.0 * (xj) * (xj)-cvsq)));
-            return rt<S, C>(P, v, l + 1, sj, k);
+            return rt<S, C>(P, v, l + 1, sjw, k);
         }
-        return get<S, C>(L, P, L.a, k, si, sj);
+        return a(k, si, sj);
     }
     // wall bounded in y: DRIVEN, CHANNEL, COUETTE
     const int o = opp_of(k);
     if (sj > m && (BC == BK_DRIVEN || BC == BK_COUETTE)) {
-        C own = get<S, C>(L, P, L.a, o, i, j);
+        C own = a(o, i, j);
         const int c = cx_of(k);
         if (c == 0) return own;
         return rt<S, C>(P, c > 0 ? own + P.force : own - P.force, si, sj, k);
     }
-    if (sj < 1 || sj > m) return get<S, C>(L, P, L.a, o, i, j);
-    if (BC == BK_DRIVEN) return get<S, C>(L, P, L.a, o, i, j);
-    si = si < 1 ? l : (si > l ? 1 : si);  // CHANNEL / COUETTE: x periodic
-    return get<S, C>(L, P, L.a, k, si, sj);
+    if (sj < 1 || sj > m) return a(o, i, j);
+    if (BC == BK_DRIVEN) return a(o, i, j);
+    return a(k, si, sj);  // CHANNEL / COUETTE: x periodic
+}
+
+template <class S, class C, int BC>
+__device__ C folded(const Lat &L, const Phys<C> &P, int k, int i, int j, int si, int sj)
+{
+    return folded_a<S, C, BC>(HbmAcc<S, C, BC>{L, P}, P, L.l, L.m, k, i, j, si, sj);
 }
 
 // Replace the pulled values whose source is a halo node (folded boundary)
 // or a solid node (bcond_obs), in that order, as the reference applies the
-// wall boundary before the obstacle one.
-template <class S, class C, int BC, bool OBS>
-__device__ __forceinline__ void fix_pulls(const Lat &L, const Phys<C> &P, int i, int j, C x[9])
+// wall boundary before the obstacle one.  With WRAPPED the caller's pulls
+// already hold the wrapped values along periodic directions, so only the
+// sources outside a wall-bounded direction are replaced.
+template <class S, class C, int BC, bool OBS, bool WRAPPED = false, class A>
+__device__ __forceinline__ void fix_pulls_a(const A &a, const Lat &L, const Phys<C> &P, int i, int j, C x[9])
 {
-    if (i == 1 || i == L.l || j == 1 || j == L.m) {
+    constexpr bool fx = !(WRAPPED && Per<BC>::x), fy = !(WRAPPED && Per<BC>::y);
+    if ((fx && (i == 1 || i == L.l)) || (fy && (j == 1 || j == L.m))) {
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             int si = i - cx_of(k), sj = j - cy_of(k);
-            if (si < 1 || si > L.l || sj < 1 || sj > L.m) x[k] = folded<S, C, BC>(L, P, k, i, j, si, sj);
+            if ((fx && (si < 1 || si > L.l)) || (fy && (sj < 1 || sj > L.m)))
+                x[k] = folded_a<S, C, BC>(a, P, L.l, L.m, k, i, j, si, sj);
         }
     }
     if (OBS && i >= L.imin - 1 && i <= L.imax + 1 && j >= L.jmin - 1 && j <= L.jmax + 1) {
@@ -424,9 +453,15 @@ __device__ __forceinline__ void fix_pulls(const Lat &L, const Phys<C> &P, int i,
             int si = i - cx_of(k), sj = j - cy_of(k);
             // the pulled value was set by bcond_obs: keep it only when this
             // node is solid and precedes the source in the serial order
-            if (solid(L, si, sj) && !(me && before_pop(k))) x[k] = get<S, C>(L, P, L.a, opp_of(k), i, j);
+            if (solid(L, si, sj) && !(me && before_pop(k))) x[k] = a(opp_of(k), i, j);
         }
     }
+}
+
+template <class S, class C, int BC, bool OBS>
+__device__ __forceinline__ void fix_pulls(const Lat &L, const Phys<C> &P, int i, int j, C x[9])
+{
+    fix_pulls_a<S, C, BC, OBS>(HbmAcc<S, C, BC>{L, P}, L, P, i, j, x);
 }
 
 // Addressing of the hot kernel: one 32-bit byte offset per node, and per
@@ -517,6 +552,178 @@ __global__ __launch_bounds__(256) void k_fused2(Lat L, Phys<C> P)
             __builtin_nontemporal_store(ow[k], (uint32_t *)dst);
         else
             __builtin_nontemporal_store((uint16_t)(ow[k] & 0xffffu), (uint16_t *)dst);
+    }
+}
+
+// ------------------------------------------------ two steps per HBM pass
+// k_fused_tb advances the lattice by two FUSED steps in one pass over HBM.
+// A block owns a strip of TB_TW columns and H rows and marches up the strip
+// one row r at a time, thread t owning column i0+t in both phases:
+//   phase 1: step t+1 of row r, pulled from HBM (boundary folded),
+//            collided and rounded to the storage kind;
+//   phase 2: step t+2 of row r-1, pulled from the step t+1 values, stored
+//            to HBM.
+// The six populations that move along x (01,03,05,10,12,14) pass between
+// threads through an LDS ring of four rows, which also holds the rim
+// columns i0-1 (pops 01,03,05) and i0+TW (pops 10,12,14) that the strip's
+// edge nodes pull; 08, 17 and 19 stay in the owning thread's registers as
+// a three-row history.  The rim columns' step t+1 values are computed
+// TB_RB rows at a time (a few wavefront passes instead of one per row) and
+// copied into the ring row by row.  The raw pulls of row r+1 are issued
+// before phase 2 of row r-1, so their latency overlaps it.
+// Along periodic directions the rim columns and the rows above/below the
+// domain hold the wrapped nodes, so phase 2 pulls them directly; along
+// wall-bounded directions phase 2 folds the boundary from step t+1 values
+// of its own column (own node; the column-l outflow of INFLOW).
+// Every value equals what two k_fused launches would store: each node is
+// the same function of the same rounded inputs (rim nodes are computed by
+// both blocks that read them).  Pop 19 moves from the plane L.r to the
+// plane L.rn, because neighbouring blocks still read the old value of a
+// node when its owner writes the new one.
+// Ring hazard: phase 2 of row r-1 reads ring rows r-2..r; phase 1 of row
+// r+1 overwrites ring row r-3, read last by phase 2 of row r-2, which every
+// thread has finished when it passes the barrier of row r.
+constexpr int TB_TW = 256;  // strip width (one column per thread)
+constexpr int TB_RB = 32;   // rows of rim columns computed per batch (power of two)
+
+// ring slot of population k (0..7, not 08/17): 01,03,05,10,12,14 -> 0..5
+__device__ __forceinline__ constexpr int tb_slot(int k) { return k < 3 ? k : k - 1; }
+
+template <class S, class C, int E>
+struct TbAcc {
+    using R = typename CodecT<S, C>::raw;
+    const R (*ring)[6][E];
+    int e0, r0;  // node column of ring column 0; node row of ring row 0
+    int q;       // row of the node being updated (its own column is e0+e)
+    R h08[3], h17[3], h19[3];  // rows q-1, q, q+1 of the own column
+    C inv;
+    __device__ __forceinline__ C operator()(int k, int si, int sj) const
+    {
+        const int d = sj - q + 1;
+        if (k == 3) return CodecT<S, C>::dec(d == 0 ? h08[0] : (d == 1 ? h08[1] : h08[2]), inv);
+        if (k == 7) return CodecT<S, C>::dec(d == 0 ? h17[0] : (d == 1 ? h17[1] : h17[2]), inv);
+        if (k == 8) return CodecT<S, C>::dec(d == 0 ? h19[0] : (d == 1 ? h19[1] : h19[2]), inv);
+        return CodecT<S, C>::dec(ring[(sj - r0) & 3][tb_slot(k)][si - e0], inv);
+    }
+};
+
+// step t+1 of interior node (i, j) from raw pulls, all nine populations
+template <class S, class C, int BC, bool OBS>
+__device__ __forceinline__ void tb_node(const Lat &L, const Phys<C> &P, int i, int j,
+                                        const typename CodecT<S, C>::raw xr[9], C o[9])
+{
+    C x[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) x[k] = CodecT<S, C>::dec(xr[k], P.inv);
+    fix_pulls<S, C, BC, OBS>(L, P, i, j, x);
+    relax<C>(P, x, o);
+}
+
+template <class S, class R>
+__device__ __forceinline__ void tb_load(const Lat &L, int i, int j, R xr[9])
+{
+    const uint32_t bo = ((uint32_t)j * (uint32_t)L.rs + (uint32_t)i) * (uint32_t)sizeof(S);
+#pragma unroll
+    for (int k = 0; k < 8; k++) xr[k] = *(const R *)(pull_base<S>(L, L.a[k], k) + bo);
+    xr[8] = *(const R *)(pull_base<S>(L, L.r, 8) + bo);
+}
+
+template <class S, class C, int BC, bool OBS>
+__global__ __launch_bounds__(256) void k_fused_tb(Lat L, Phys<C> P, int H)
+{
+    using R = typename CodecT<S, C>::raw;
+    constexpr int TW = TB_TW, E = TW + 2;
+    constexpr bool XP = Per<BC>::x, YP = Per<BC>::y;
+    __shared__ R ring[4][6][E];
+    constexpr int RB = TB_RB;
+    __shared__ R rim[2][RB][3];
+    const int l = L.l, m = L.m;
+    const int i0 = blockIdx.x * TW + 1, j0 = blockIdx.y * H + 1;
+    const int ncols = min(TW, l - i0 + 1), jend = min(j0 + H - 1, m);
+    const int tid = threadIdx.x;
+    // node row / column that a ring position holds, -1 for a halo of a
+    // wall-bounded direction
+    auto wrow = [&](int r) { return (r >= 1 && r <= m) ? r : (YP ? (r < 1 ? r + m : r - m) : -1); };
+    auto wcol = [&](int c) { return (c >= 1 && c <= l) ? c : (XP ? (c < 1 ? c + l : c - l) : -1); };
+    TbAcc<S, C, E> acc;
+    acc.ring = ring;
+    acc.e0 = i0 - 1;
+    acc.r0 = j0 - 1;
+    acc.inv = P.inv;
+    acc.q = j0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) acc.h08[d] = acc.h17[d] = acc.h19[d] = R{};
+    const int e = tid + 1, i = i0 + tid;
+    const bool own = tid < ncols;
+    R xr[9];
+    int wr = wrow(j0 - 1);
+    if (own && wr >= 0) tb_load<S>(L, i, wr, xr);
+    for (int r = j0 - 1; r <= jend + 1; r++) {
+        const int slot = (r - j0 + 1) & 3;
+        if (((r - j0 + 1) & (RB - 1)) == 0) {
+            // rim columns, step t+1 of the next RB rows: column i0-1 keeps
+            // pops 01,03,05 (pulled by column i0), column i0+ncols pops
+            // 10,12,14.  The previous segment's rim rows were all copied
+            // into the ring before the barrier of their own row.
+            const int nr = min(RB, jend + 2 - r);
+            for (int t = tid; t < 2 * nr; t += 256) {
+                const int side = t >= nr ? 1 : 0, rr = t - side * nr;
+                const int wi = wcol(side ? i0 + ncols : i0 - 1), wj = wrow(r + rr);
+                if (wi < 0 || wj < 0) continue;
+                R yr[9];
+                C o[9];
+                tb_load<S>(L, wi, wj, yr);
+                tb_node<S, C, BC, OBS>(L, P, wi, wj, yr, o);
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const int kk = side ? k + 4 : k;
+                    rim[side][rr][k] = encode<S, C>(P, o[kk], wi, wj, kk);
+                }
+            }
+            __syncthreads();
+        }
+        // phase 1: step t+1 of row r
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+            acc.h08[d] = acc.h08[d + 1];
+            acc.h17[d] = acc.h17[d + 1];
+            acc.h19[d] = acc.h19[d + 1];
+        }
+        if (own && wr >= 0) {
+            C o[9];
+            tb_node<S, C, BC, OBS>(L, P, i, wr, xr, o);
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (cx_of(k) != 0) ring[slot][tb_slot(k)][e] = encode<S, C>(P, o[k], i, wr, k);
+            acc.h08[2] = encode<S, C>(P, o[3], i, wr, 3);
+            acc.h17[2] = encode<S, C>(P, o[7], i, wr, 7);
+            acc.h19[2] = encode<S, C>(P, o[8], i, wr, 8);
+        }
+        if (tid < 6) {
+            const int side = tid >= 3 ? 1 : 0, k = tid - side * 3;
+            ring[slot][side ? k + 3 : k][side ? ncols + 1 : 0] = rim[side][(r - j0 + 1) & (RB - 1)][k];
+        }
+        // prefetch the raw pulls of row r+1
+        wr = r + 1 <= jend + 1 ? wrow(r + 1) : -1;
+        if (own && wr >= 0) tb_load<S>(L, i, wr, xr);
+        __syncthreads();
+        // phase 2: step t+2 of row q = r-1 (history rows r-2, r-1, r)
+        const int q = r - 1;
+        if (q < j0 || !own) continue;
+        acc.q = q;
+        C x[9], o[9];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (cx_of(k) != 0) x[k] = CodecT<S, C>::dec(ring[(q - cy_of(k) - j0 + 1) & 3][tb_slot(k)][e - cx_of(k)], P.inv);
+        x[3] = CodecT<S, C>::dec(acc.h08[0], P.inv);  // 08 from (i, q-1)
+        x[7] = CodecT<S, C>::dec(acc.h17[2], P.inv);  // 17 from (i, q+1)
+        x[8] = CodecT<S, C>::dec(acc.h19[1], P.inv);
+        fix_pulls_a<S, C, BC, OBS, true>(acc, L, P, i, q, x);
+        relax<C>(P, x, o);
+        const uint32_t bo = ((uint32_t)q * (uint32_t)L.rs + (uint32_t)i) * (uint32_t)sizeof(S);
+#pragma unroll
+        for (int k = 0; k < 8; k++) st_nt((R *)(pull_base<S>(L, L.b[k], 8) + bo), encode<S, C>(P, o[k], i, q, k));
+        st_nt((R *)(pull_base<S>(L, L.rn, 8) + bo), encode<S, C>(P, o[8], i, q, 8));
     }
 }
 

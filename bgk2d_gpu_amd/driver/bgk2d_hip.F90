@@ -76,7 +76,7 @@ program bgk2d_hip
   p%mixed1_quirk = 0
   p%form = form
   p%rounding = rounding
-  p%reserved = 0
+  p%tblock = 0
   p%obstacle = 0
   p%scale = scale
   select case (trim(case_name))

@@ -52,11 +52,14 @@ class Request:
         lx, ly = self.geometry()[:2]
         return lx * ly * self.steps
 
-    def params(self, device=0, form=1, rounding=1):
+    def params(self, device=0, form=1, rounding=1, tblock=1):
+        """bgk_params_t of the solve.  tblock=1 (one step per kernel) is the
+        step the batch kernel performs, so sequential and batched serving
+        run the same per-node code."""
         lx, ly, bc, svisc, u0, _ = self.geometry()
         narrow = self.prec in ("fp16", "bf16")
         return BgkParams(lx, ly, 0, 0, BC[bc], 0, IMPL["fused"], PREC[self.prec], 16, device, 0,
-                         form if narrow else 0, rounding if narrow else 0, 0, float(svisc),
+                         form if narrow else 0, rounding if narrow else 0, tblock, float(svisc),
                          float(u0), 1024.0 if narrow else 1.0)
 
 

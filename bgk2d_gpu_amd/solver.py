@@ -35,7 +35,7 @@ class BgkParams(C.Structure):
                 ("bc", C.c_int32), ("obstacle", C.c_int32), ("impl", C.c_int32),
                 ("prec", C.c_int32), ("radius", C.c_int32), ("device", C.c_int32),
                 ("mixed1_quirk", C.c_int32), ("form", C.c_int32),
-                ("rounding", C.c_int32), ("reserved", C.c_int32),
+                ("rounding", C.c_int32), ("tblock", C.c_int32),
                 ("svisc", C.c_double), ("u0", C.c_double), ("scale", C.c_double)]
 
 
@@ -102,7 +102,7 @@ class Solver:
 
     def __init__(self, lx, ly, bc="driven", impl="fused", prec="single", svisc=0.1, u0=0.1,
                  obstacle=0, radius=16, scale=1.0, device=0, mixed1_quirk=0, init=True,
-                 init_v=0, amplitude=0.1, form=0, rounding=0):
+                 init_v=0, amplitude=0.1, form=0, rounding=0, tblock=0):
         self.L = load()
         self.lx, self.ly = int(lx), int(ly)
         self.prec = prec
@@ -110,7 +110,7 @@ class Solver:
         self.impl = impl
         self.p = BgkParams(self.lx, self.ly, 0, 0, BC[bc], int(obstacle), IMPL[impl], PREC[prec],
                            int(radius), int(device), int(mixed1_quirk), int(form),
-                           int(rounding), 0, float(svisc),
+                           int(rounding), int(tblock), float(svisc),
                            float(u0), float(scale))
         h = C.c_void_p()
         rc = self.L.bgk_create(C.byref(self.p), C.byref(h))

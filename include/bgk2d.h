@@ -89,7 +89,10 @@ typedef struct bgk_params {
     int32_t rounding;     /* FP16/BF16 stores: 0 round to nearest even (the
                              reference's MIXED1); 1 deterministic stochastic
                              rounding (hash of value and position)          */
-    int32_t reserved;     /* must be 0                                      */
+    int32_t tblock;       /* FUSED steps per pass over HBM in bgk_run: 0 the
+                             library's choice (2 for fp32/fp64 storage, 1
+                             for FP16/BF16), 1 one step per kernel, 2 two
+                             steps per kernel; results are identical       */
     double svisc;         /* namelist svisc (kinematic viscosity)           */
     double u0;            /* namelist u0 (lid speed, or channel force scale)*/
     double scale;         /* FP16/BF16 storage scale (power of two, >= 1)   */
@@ -97,7 +100,8 @@ typedef struct bgk_params {
 
 typedef struct bgk_ctx bgk_ctx_t;
 
-/* Context: allocates the two population sets in HBM (17 pitched planes),
+/* Context: allocates the two population sets in HBM (18 pitched planes:
+ * eight moving populations per set and two planes of the rest population),
  * fills them with huge(mykind) as alloca does, and derives omega, fgrad
  * and the lid speed in the compute kind exactly as hencol/build_bcond. */
 int bgk_create(const bgk_params_t *p, bgk_ctx_t **out);
@@ -130,10 +134,10 @@ int bgk_propagation(bgk_ctx_t *ctx);
 int bgk_collision(bgk_ctx_t *ctx, int32_t itime);
 
 /* nsteps complete steps.  FUSED with a foldable boundary (all but STEP)
- * runs one kernel per step with the boundary folded into the pull; the
- * rest replay boundaries/propagation/collision.  Steps are captured into
- * HIP graphs in chunks.  Interior results are identical to the per-phase
- * entry points.                                                          */
+ * runs the boundary folded into the pull, one or two steps per kernel
+ * (params.tblock); the rest replay boundaries/propagation/collision.
+ * Steps are captured into HIP graphs in chunks.  Interior results are
+ * identical to the per-phase entry points.                               */
 int bgk_run(bgk_ctx_t *ctx, int32_t nsteps);
 int bgk_sync(bgk_ctx_t *ctx);
 
@@ -155,8 +159,8 @@ int bgk_upload(bgk_ctx_t *ctx, int32_t k, const void *host);
 int bgk_storage_bytes(const bgk_ctx_t *ctx);
 
 /* Stream and timing helpers for benchmarks: the context stream as an
- * opaque handle, and the average duration (ms) of the main kernel of the
- * last bgk_run measured with HIP events on that stream.                  */
+ * opaque handle, and the average time per step (ms) of the last bgk_run,
+ * measured with HIP events on that stream around all of its launches.    */
 void *bgk_stream(bgk_ctx_t *ctx);
 int bgk_last_run_ms(const bgk_ctx_t *ctx, double *ms);
 

@@ -96,16 +96,16 @@ def test_against_oracle(case, prec):
                   obstacle=obst, radius=9, scale=scale)
     o.step(ns)
     ref = o.raw()
-    for impl in ("fused", "original"):
+    for impl, tb in (("fused", 1), ("fused", 2), ("original", 0)):
         s = Solver(lx, ly, bc=bc, impl=impl, prec=prec, svisc=sv, u0=u0, obstacle=obst,
-                   radius=9, scale=scale)
+                   radius=9, scale=scale, tblock=tb)
         s.run(ns)
         got = s.raw()
         inner = (slice(None), slice(1, -1), slice(1, -1))
         ok = same_bits(got[inner], ref[inner], prec)
         if bc == "step" and impl == "original":
             ok = ok[:, ly // 2:, :]  # FUSED reads never-written halos inside the step
-        assert ok.all(), f"{impl}: {int((~ok).sum())} differing elements"
+        assert ok.all(), f"{impl}/tblock={tb}: {int((~ok).sum())} differing elements"
         if impl == "fused":
             d, dref = s.diagno(), o.diagno(mode=0)
             np.testing.assert_allclose(d, dref, rtol=1e-12, atol=1e-17)
@@ -141,17 +141,53 @@ def test_options_against_oracle(case):
                   init_v=init_v, amplitude=0.05)
     o.step(ns)
     ref = o.raw()
-    for impl in ("fused", "original"):
+    for impl, tb in (("fused", 1), ("fused", 2), ("original", 0)):
         s = Solver(lx, ly, bc=bc, impl=impl, prec=prec, svisc=0.05, u0=0.05, obstacle=obst,
                    radius=9, scale=64.0, form=form, rounding=rounding, init_v=init_v,
-                   amplitude=0.05)
+                   amplitude=0.05, tblock=tb)
         s.run(ns)
         got = s.raw()
         ok = same_bits(got[:, 1:-1, 1:-1], ref[:, 1:-1, 1:-1], prec)
         if bc == "step" and impl == "original":
             ok = ok[:, ly // 2:, :]
-        assert ok.all(), f"{impl}: {int((~ok).sum())} differing elements"
+        assert ok.all(), f"{impl}/tblock={tb}: {int((~ok).sum())} differing elements"
         s.close()
+
+
+TB_CASES = [
+    # (bc, obstacle, lx, ly): several strips of 256 columns, a partial last
+    # strip, and heights that do not divide into the strip rows
+    ("driven", 0, 600, 301),
+    ("periodic", 0, 1030, 77),
+    ("channel", 0, 513, 130),
+    ("inflow", 1, 700, 200),
+    ("couette", 0, 300, 97),
+    ("driven", 1, 257, 513),
+]
+
+
+@pytest.mark.parametrize("prec", ["single", "double", "fp16", "bf16", "mixed2"])
+@pytest.mark.parametrize("case", TB_CASES, ids=lambda c: f"{c[0]}{'+obs' if c[1] else ''}-{c[2]}x{c[3]}")
+def test_two_steps_per_pass_equal_one(case, prec):
+    """k_fused_tb (two steps per HBM pass) against k_fused (one step per
+    pass), bit for bit, through graph replays, an odd remainder step and
+    diagnostics in between"""
+    bc, obst, lx, ly = case
+    print(f"case {bc} obs={obst} {lx}x{ly} {prec}", flush=True)
+    kw = dict(bc=bc, prec=prec, svisc=0.03, u0=0.08, obstacle=obst, radius=11,
+              scale=256.0 if prec in ("fp16", "bf16") else 1.0,
+              form=1 if prec in ("fp16", "bf16") else 0,
+              rounding=1 if prec in ("fp16", "bf16") else 0)
+    a = Solver(lx, ly, tblock=1, **kw)
+    b = Solver(lx, ly, tblock=2, **kw)
+    for n in (67, 3, 2):
+        a.run(n)
+        b.run(n)
+        np.testing.assert_allclose(a.diagno(), b.diagno(), rtol=0, atol=0)
+        ok = same_bits(a.raw()[:, 1:-1, 1:-1], b.raw()[:, 1:-1, 1:-1], prec)
+        assert ok.all(), f"after {n}: {int((~ok).sum())} differing elements"
+    a.close()
+    b.close()
 
 
 def test_phases_equal_run_fused_driven_bitexact():
