@@ -32,6 +32,7 @@ struct bgk_ctx {
     int cur;          // which set is `a`
     int rcur;         // which pop-19 plane is current (16 + rcur)
     int tb_h;         // strip height of the two-step kernel (0: not sized yet)
+    bool tb_fill;     // the lattice fills the device with strips of >= TB_HMIN rows
     uint8_t *d_obs;
     int imin, imax, jmin, jmax, nobs;
     std::vector<int32_t> h_obs;
@@ -41,6 +42,7 @@ struct bgk_ctx {
     // constants in the compute kind, widened
     double omega, fgrad, u00, force, scale;
     double inflow[3];
+    double umax;  // OPEN: peak inflow speed in the compute kind
     bool inited;
     std::string err;
     hipEvent_t ev0, ev1;
@@ -88,8 +90,12 @@ void derive(bgk_ctx *c)
 {
     const bgk_params_t &p = c->p;
     C u0 = (C)p.u0, u00;
+    c->umax = p.bc == BGK_BC_OPEN ? (double)u0 : 0.0;
     if (p.bc == BGK_BC_CHANNEL) {
         u00 = (C)0.0;
+    } else if (p.bc == BGK_BC_OPEN) {  // no lid and no body force: u0 is the inflow peak
+        u00 = (C)0.0;
+        u0 = (C)0.0;
     } else {
         u00 = u0;
         u0 = (C)0.0;
@@ -121,6 +127,7 @@ Phys<C> phys(const bgk_ctx *c)
     P.sc = (C)c->scale;
     P.inv = (C)(1.0 / c->scale);
     for (int q = 0; q < 3; q++) P.inflow[q] = (C)c->inflow[q];
+    P.umax = (C)c->umax;
     P.channel_force = c->p.bc == BGK_BC_CHANNEL ? 1 : 0;
     P.form = c->p.form;
     P.rounding = (is_fp16(c->p.prec) || is_bf16(c->p.prec)) ? c->p.rounding : 0;
@@ -183,6 +190,10 @@ int launch_boundaries(bgk_ctx *c)
         pass(BP_XPER, 1, m, 0);
         pass(BP_WALL_Y, 1, l, 1);
         break;
+    case BGK_BC_OPEN:
+        pass(BP_OPEN_X, 1, m, 0);
+        pass(BP_FREESLIP_Y, 0, l + 2, 0);
+        break;
     case BGK_BC_STEP:
         pass(BP_STEP_OUT, 0, m + 2, 0);
         pass(BP_STEP_IN, L.stepy, m + 2 - L.stepy, 0);
@@ -242,6 +253,7 @@ void launch_fused_t(bgk_ctx *c)
         hipLaunchKernelGGL((k_fused<S, C, BC, OBS>), grid2(c->l, c->m, B2), B2, 0, c->s, L, P);
     }
     c->cur = 1 - c->cur;
+    c->rcur = 1 - c->rcur;  // pop 19 written to the second plane
 }
 
 template <class S, class C, bool OBS>
@@ -252,6 +264,7 @@ void launch_fused_o(bgk_ctx *c)
     case BGK_BC_CHANNEL: launch_fused_t<S, C, BK_CHANNEL, OBS>(c); break;
     case BGK_BC_INFLOW: launch_fused_t<S, C, BK_INFLOW, OBS>(c); break;
     case BGK_BC_COUETTE: launch_fused_t<S, C, BK_COUETTE, OBS>(c); break;
+    case BGK_BC_OPEN: launch_fused_t<S, C, BK_OPEN, OBS>(c); break;
     default: launch_fused_t<S, C, BK_DRIVEN, OBS>(c); break;
     }
 }
@@ -274,18 +287,22 @@ int launch_step(bgk_ctx *c)
 }
 
 // Two FUSED steps per pass over HBM (k_fused_tb): the default for fp32 and
-// fp64 storage, where it beats one step per pass; for FP16/BF16 storage
-// the paired one-step kernel is as fast, so it stays the default there.
+// fp64 storage on lattices large enough to fill the device with strips of
+// at least TB_HMIN rows, where it beats one step per pass; for FP16/BF16
+// storage the paired one-step kernel is as fast, so it stays the default
+// there.  Needs size_tb first.
+constexpr int TB_HMIN = 8;
 bool use_tb(const bgk_ctx *c)
 {
     if (!foldable(c)) return false;
     if (c->p.tblock == 2) return true;
-    return c->p.tblock == 0 && c->esize >= 4;
+    return c->p.tblock == 0 && c->esize >= 4 && c->tb_fill;
 }
 
 // Strip height of k_fused_tb: as many strips of rows as make the grid one
 // round of the blocks the device holds at once, so no CU idles in a
-// partial last round.
+// partial last round; at least TB_HMIN rows, since each strip also
+// computes two rows of its neighbours.
 template <class S, class C, int BC, bool OBS>
 int size_tb(bgk_ctx *c)
 {
@@ -298,7 +315,9 @@ int size_tb(bgk_ctx *c)
     const long long strips = (c->l + TB_TW - 1) / TB_TW;
     long long per_strip = slots / strips;
     if (per_strip < 1) per_strip = 1;
-    c->tb_h = (int)((c->m + per_strip - 1) / per_strip);
+    const int h = (int)((c->m + per_strip - 1) / per_strip);
+    c->tb_fill = h >= TB_HMIN;
+    c->tb_h = h >= TB_HMIN ? h : TB_HMIN;
     return BGK_OK;
 }
 
@@ -310,6 +329,7 @@ int size_tb_o(bgk_ctx *c)
     case BGK_BC_CHANNEL: return size_tb<S, C, BK_CHANNEL, OBS>(c);
     case BGK_BC_INFLOW: return size_tb<S, C, BK_INFLOW, OBS>(c);
     case BGK_BC_COUETTE: return size_tb<S, C, BK_COUETTE, OBS>(c);
+    case BGK_BC_OPEN: return size_tb<S, C, BK_OPEN, OBS>(c);
     default: return size_tb<S, C, BK_DRIVEN, OBS>(c);
     }
 }
@@ -339,6 +359,7 @@ int launch_tb_o(bgk_ctx *c)
     case BGK_BC_CHANNEL: return launch_tb_t<S, C, BK_CHANNEL, OBS>(c);
     case BGK_BC_INFLOW: return launch_tb_t<S, C, BK_INFLOW, OBS>(c);
     case BGK_BC_COUETTE: return launch_tb_t<S, C, BK_COUETTE, OBS>(c);
+    case BGK_BC_OPEN: return launch_tb_t<S, C, BK_OPEN, OBS>(c);
     default: return launch_tb_t<S, C, BK_DRIVEN, OBS>(c);
     }
 }
@@ -377,11 +398,6 @@ int graph_for(bgk_ctx *c, hipGraphExec_t *out)
         *out = it->second;
         return BGK_OK;
     }
-    if (use_tb(c) && c->tb_h == 0) {
-        // size the two-step kernel before the capture
-        int r = c->p.obstacle && c->nobs > 0 ? size_tb_o<S, C, true>(c) : size_tb_o<S, C, false>(c);
-        if (r != BGK_OK) return r;
-    }
     const int cur0 = c->cur, rcur0 = c->rcur;
     HIPCHK(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
     int r = launch_steps<S, C>(c, GRAPH_STEPS);
@@ -406,6 +422,12 @@ int graph_for(bgk_ctx *c, hipGraphExec_t *out)
 template <class S, class C>
 int run_t(bgk_ctx *c, int nsteps)
 {
+    if (foldable(c) && c->tb_h == 0) {
+        // size the two-step kernel (which decides whether it is used)
+        // before any capture
+        int r = c->p.obstacle && c->nobs > 0 ? size_tb_o<S, C, true>(c) : size_tb_o<S, C, false>(c);
+        if (r != BGK_OK) return r;
+    }
     HIPCHK(c, hipEventRecord(c->ev0, c->s));
     int t = 0;
     if (nsteps >= GRAPH_STEPS) {
@@ -484,6 +506,16 @@ int line_t(bgk_ctx *c, int mode, int c0, int c1, int n)
     Lat L = lat(c);
     Phys<C> P = phys<C>(c);
     hipLaunchKernelGGL((k_line<S, C>), dim3((n + 255) / 256), dim3(256), 0, c->s, L, P, mode, c0, c1, c->d_red);
+    HIPCHK(c, hipGetLastError());
+    return BGK_OK;
+}
+
+template <class S, class C>
+int draglift_t(bgk_ctx *c, int i0, int i1, int j0, int j1)
+{
+    Lat L = lat(c);
+    Phys<C> P = phys<C>(c);
+    hipLaunchKernelGGL((k_draglift<S, C>), dim3(1), dim3(DL_T), 0, c->s, L, P, i0, i1, j0, j1, c->d_red);
     HIPCHK(c, hipGetLastError());
     return BGK_OK;
 }
@@ -578,7 +610,7 @@ int bgk_create(const bgk_params_t *p, bgk_ctx_t **out)
     if (!p || !out) return BGK_E_INVALID;
     *out = nullptr;
     if (p->lx < 2 || p->ly < 2 || p->lx > (1 << 20) || p->ly > (1 << 20)) return BGK_E_INVALID;
-    if (p->bc < 0 || p->bc > 5 || p->prec < 0 || p->prec > 4 || (p->impl != 0 && p->impl != 1))
+    if (p->bc < 0 || p->bc > 6 || p->prec < 0 || p->prec > 4 || (p->impl != 0 && p->impl != 1))
         return BGK_E_INVALID;
     if (p->bc == BGK_BC_STEP && p->lx < 20) return BGK_E_INVALID;  // stepx = l/20 must be >= 1
     if (!(p->svisc > 0.0)) return BGK_E_INVALID;
@@ -609,6 +641,7 @@ int bgk_create(const bgk_params_t *p, bgk_ctx_t **out)
     c->cur = 0;
     c->rcur = 0;
     c->tb_h = 0;
+    c->tb_fill = false;
     c->last_ms = 0.0;
     c->inited = false;
     c->stage_bytes = 0;
@@ -719,6 +752,8 @@ static int set_mask(bgk_ctx *c, const int32_t *obs)
     c->jmin = jmin;
     c->jmax = jmax;
     c->nobs = n;
+    c->p.obstacle = n > 0 ? 1 : 0;  // a mask turns bcond_obs on, as the OBSTACLE build
+    c->tb_h = 0;                    // the two-step kernel is sized per instance
     c->h_obs.assign(obs, obs + (size_t)c->l * c->m);
     HIPCHK(c, hipMemcpyAsync(c->d_obs, h.data(), h.size(), hipMemcpyHostToDevice, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
@@ -902,6 +937,25 @@ int bgk_probe(bgk_ctx_t *c, int32_t i0, int32_t j0, double out[3])
 {
     if (!c || !out || i0 < 0 || i0 > c->l + 1 || j0 < 0 || j0 > c->m + 1) return BGK_E_INVALID;
     return line_common(c, 2, i0, j0, 1, &out[0], &out[1], &out[2]);
+}
+
+int bgk_draglift(bgk_ctx_t *c, int32_t i0, int32_t i1, int32_t j0, int32_t j1, double out[2])
+{
+    if (!c || !out) return BGK_E_INVALID;
+    int r = check_ctx(c);
+    if (r) return r;
+    // the box is clipped to the interior; outside it no node is solid
+    i0 = i0 < 1 ? 1 : i0;
+    j0 = j0 < 1 ? 1 : j0;
+    i1 = i1 > c->l ? c->l : i1;
+    j1 = j1 > c->m ? c->m : j1;
+    if (i0 > i1 || j0 > j1) return fail(c, BGK_E_INVALID, "draglift: empty box");
+    r = DISPATCH(c, draglift_t, i0, i1, j0, j1);
+    if (r == BGK_OK) r = fetch_red(c, 2);
+    if (r) return r;
+    out[0] = c->h_red[0];
+    out[1] = c->h_red[1];
+    return BGK_OK;
 }
 
 int bgk_storage_bytes(const bgk_ctx_t *c) { return c ? c->esize : BGK_E_INVALID; }
@@ -1203,7 +1257,9 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
             d.set[0][k] = c->plane[k];
             d.set[1][k] = c->plane[8 + k];
         }
-        d.r = c->plane[16 + c->rcur];
+        d.r[0] = c->plane[16];
+        d.r[1] = c->plane[17];
+        d.rparity0 = c->rcur;
         d.l = c->l;
         d.m = c->m;
         d.off = c->off;
@@ -1252,6 +1308,7 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
         auto &sl = b->slots[members_slot[mi]];
         const int steps = b->h_members[mi].remaining;
         sl.ctx->cur = (sl.ctx->cur + steps) & 1;
+        sl.ctx->rcur = (sl.ctx->rcur + steps) & 1;
         sl.remaining -= steps;
         sl.done += steps;
         if (sl.remaining == 0) nfin++;

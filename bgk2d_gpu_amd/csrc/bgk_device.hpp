@@ -148,6 +148,7 @@ struct Phys {
     C omega, fgrad, force;  // hencol / bcond_driven constants
     C sc, inv;              // storage scale and its inverse (FP16/BF16)
     C inflow[3];            // stored-and-decoded inflow equilibria 01,03,05
+    C umax;                 // OPEN: peak of the parabolic inflow profile
     int channel_force;      // 1: CHANNEL body force in collision
     int form;               // 0: reference operation order; 1: deviation form
     int rounding;           // FP16/BF16 stores: 0 nearest-even, 1 stochastic

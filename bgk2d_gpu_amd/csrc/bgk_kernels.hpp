@@ -147,8 +147,45 @@ enum BcPass {
     BP_STEP_OUT = 5, // outflow only, j = 0..m+1        SRC/bcond.step.F90:66-102
     BP_STEP_IN = 6,  // parabolic inflow, j = stepy..m+1 SRC/bcond.step.F90:116-151
     BP_STEP_TOP = 7, // step top face, i = 1..stepx     SRC/bcond.step.F90:205-223
-    BP_STEP_FACE = 8 // step vertical face, j = 1..stepy SRC/bcond.step.F90:237-255
+    BP_STEP_FACE = 8, // step vertical face, j = 1..stepy SRC/bcond.step.F90:237-255
+    BP_OPEN_X = 9,    // OPEN: parabolic inflow + zero-gradient outflow, j = 1..m
+    BP_FREESLIP_Y = 10 // OPEN: specular (free-slip) walls, i = 0..l+1
 };
+
+// OPEN inflow profile: u(j) = 4 umax y (1 - y), y = (j - 1/2) / m, the
+// parabola through the two halfway walls
+template <class C>
+__device__ __host__ __forceinline__ C open_u(const Phys<C> &P, int j, int m)
+{
+    const C y = ((C)j - (C)0.5) / (C)m;
+    return (C)4.0 * P.umax * y * ((C)1.0 - y);
+}
+// stored value of inflow population k (0..2: 01,03,05) at x = 0, row j:
+// the shifted equilibrium of u = (open_u(j), 0) at the density 1 + drho of
+// the first fluid node (1, j), rho w_k eq_k + w_k drho -- a velocity inlet
+// whose density follows the interior
+template <class C>
+__device__ __forceinline__ C open_in(const Phys<C> &P, int k, int j, int m, C drho)
+{
+    const C rho = (C)1.0 + drho, yj = (C)0.0, xj = open_u<C>(P, j, m);
+    const C cvsq = xj * xj + yj * yj;
+    if (k == 0) return rho * K<C>::p2 * eq_d<C>(xj - yj, cvsq) + K<C>::p2 * drho;
+    if (k == 1) return rho * K<C>::p2 * eq_d<C>(xj + yj, cvsq) + K<C>::p2 * drho;
+    return rho * K<C>::p1 * eq_d<C>(xj, cvsq) + K<C>::p1 * drho;
+}
+// density deviation of a node from its nine populations, in the collision's
+// summation order
+template <class C, class A>
+__device__ __forceinline__ C drho_of(const A &a, int i, int j)
+{
+    return ((a(0, i, j) + a(1, i, j) + a(4, i, j) + a(5, i, j)) + (a(2, i, j) + a(3, i, j) + a(6, i, j) + a(7, i, j))) +
+           a(8, i, j);
+}
+// specular image across a horizontal wall: 01<->03, 08<->17, 10<->12
+__device__ __forceinline__ constexpr int mirror_y(int k)
+{
+    return k == 0 ? 1 : k == 1 ? 0 : k == 3 ? 7 : k == 7 ? 3 : k == 4 ? 5 : k == 5 ? 4 : k;
+}
 
 template <class S, class C>
 __device__ void outflow_col(const Lat &L, const Phys<C> &P, int j)
@@ -252,6 +289,25 @@ __global__ void k_bc(Lat L, Phys<C> P, int pass, int t0, int n, int lid)
         cpy<S>(L, a[5], i + 1, sy - 1, a[0], i, sy);
         cpy<S>(L, a[1], i - 1, sy - 1, a[4], i, sy);
     } break;
+    case BP_OPEN_X: {
+        int j = t;
+        auto g = [&](int k, int si, int sj) { return get<S, C>(L, P, a, k, si, sj); };
+        const C drho = drho_of<C>(g, 1, j);
+#pragma unroll
+        for (int k = 0; k < 3; k++) put<S, C>(L, P, a, k, 0, j, open_in<C>(P, k, j, m, drho));
+        cpy<S>(L, a[4], l1, j, a[4], l, j);  // zero gradient: column l+1 = column l
+        cpy<S>(L, a[5], l1, j, a[5], l, j);
+        cpy<S>(L, a[6], l1, j, a[6], l, j);
+    } break;
+    case BP_FREESLIP_Y: {
+        int i = t;
+        cpy<S>(L, a[1], i, 0, a[0], i, 1);   // a03(i,0)  = a01(i,1)
+        cpy<S>(L, a[3], i, 0, a[7], i, 1);   // a08(i,0)  = a17(i,1)
+        cpy<S>(L, a[5], i, 0, a[4], i, 1);   // a12(i,0)  = a10(i,1)
+        cpy<S>(L, a[0], i, m1, a[1], i, m);  // a01(i,m1) = a03(i,m)
+        cpy<S>(L, a[7], i, m1, a[3], i, m);  // a17(i,m1) = a08(i,m)
+        cpy<S>(L, a[4], i, m1, a[5], i, m);  // a10(i,m1) = a12(i,m)
+    } break;
     case BP_STEP_FACE: {
         int j = t;
         const int sx = L.stepx;
@@ -354,7 +410,13 @@ __global__ void k_colmc(Lat L, Phys<C> P)
 //   COUETTE : source row m+1 -> lid; row 0 -> own opposite; else wrap x.
 //   INFLOW  : wrap y first; column 0 -> uniform inflow equilibrium,
 //             column l+1 -> equilibrium of the momentum of (l, row).
-enum BcKind { BK_DRIVEN = 0, BK_PERIODIC = 1, BK_CHANNEL = 2, BK_INFLOW = 3, BK_COUETTE = 4 };
+//   OPEN    : row 0 / m+1 -> the mirror population (01<->03, 08<->17,
+//             10<->12) of the same column in row 1 / m, that column's value
+//             after the x pass; column 0 -> parabolic inflow equilibrium at
+//             the density of (1, row); column l+1 -> the same population
+//             of (l, row).
+// BK_* shares the numbering of the foldable BGK_BC_* variants (5 is STEP).
+enum BcKind { BK_DRIVEN = 0, BK_PERIODIC = 1, BK_CHANNEL = 2, BK_INFLOW = 3, BK_COUETTE = 4, BK_OPEN = 6 };
 
 // Periodic directions of each foldable variant
 template <int BC>
@@ -385,6 +447,16 @@ template <class S, class C, int BC, class A>
 __device__ C folded_a(const A &a, const Phys<C> &P, int l, int m, int k, int i, int j, int si, int sj)
 {
     if (BC == BK_PERIODIC) return a(k, si, sj);
+    if (BC == BK_OPEN) {
+        int kk = k, rj = sj;
+        if (sj < 1 || sj > m) {  // the y pass copies the mirror population of row 1 / m
+            kk = mirror_y(k);
+            rj = sj < 1 ? 1 : m;
+        }
+        if (si < 1) return rt<S, C>(P, open_in<C>(P, kk, rj, m, drho_of<C>(a, 1, rj)), 0, rj, kk);
+        if (si > l) return a(kk, l, rj);
+        return a(kk, si, rj);
+    }
     if (BC == BK_INFLOW) {
         const int sjw = sj < 1 ? m : (sj > m ? 1 : sj);  // the y pass runs after the x pass
         if (si < 1) return rt<S, C>(P, P.inflow[k], 0, sjw, k);  // pops 01,03,05 from x = 0
@@ -475,7 +547,9 @@ __device__ __forceinline__ const char *pull_base(const Lat &L, const void *plane
 }
 
 // one FUSED step of interior node (i, j): pull (boundary folded), relax,
-// store into the other set (and the rest population in place)
+// store into the other set.  The rest population goes to the second pop-19
+// plane rather than in place: the OPEN inflow reads the rest population of
+// the neighbouring rows' first column, which must still be the old one.
 template <class S, class C, int BC, bool OBS>
 __device__ __forceinline__ void fused_node(const Lat &L, const Phys<C> &P, int i, int j)
 {
@@ -491,7 +565,7 @@ __device__ __forceinline__ void fused_node(const Lat &L, const Phys<C> &P, int i
     relax<C>(P, x, o);
 #pragma unroll
     for (int k = 0; k < 8; k++) st_nt((R *)(pull_base<S>(L, L.b[k], 8) + bo), encode<S, C>(P, o[k], i, j, k));
-    st_nt((R *)(pull_base<S>(L, L.r, 8) + bo), encode<S, C>(P, o[8], i, j, 8));
+    st_nt((R *)(pull_base<S>(L, L.rn, 8) + bo), encode<S, C>(P, o[8], i, j, 8));
 }
 
 template <class S, class C, int BC, bool OBS>
@@ -547,7 +621,7 @@ __global__ __launch_bounds__(256) void k_fused2(Lat L, Phys<C> P)
     }
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        char *dst = (char *)pull_base<S>(L, k == 8 ? L.r : L.b[k], 8) + bo;
+        char *dst = (char *)pull_base<S>(L, k == 8 ? L.rn : L.b[k], 8) + bo;
         if (two)
             __builtin_nontemporal_store(ow[k], (uint32_t *)dst);
         else
@@ -854,6 +928,45 @@ __global__ void k_line(Lat L, Phys<C> P, int mode, int c0, int c1, double *out)
 }
 
 // population k (0..8) <-> dense (l+2)*(m+2) array, x fastest
+// Momentum-exchange force on the obstacle (SRC/draglift.F90:58-98): over
+// the fluid nodes of the box [i0,i1] x [j0,j1], 2 c_k a_k(i,j) for every
+// population k whose neighbour (i,j)+c_k is solid; x and y components,
+// per-link terms in the compute kind, summed in FP64 by one block in a
+// fixed order.
+constexpr int DL_T = 256;
+template <class S, class C>
+__global__ __launch_bounds__(DL_T) void k_draglift(Lat L, Phys<C> P, int i0, int i1, int j0, int j1, double *out)
+{
+    __shared__ double sx[DL_T], sy[DL_T];
+    const int w = i1 - i0 + 1, n = w * (j1 - j0 + 1);
+    double fx = 0.0, fy = 0.0;
+    for (int t = threadIdx.x; t < n; t += DL_T) {
+        const int i = i0 + t % w, j = j0 + t / w;
+        if (solid(L, i, j)) continue;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            if (!solid(L, i + cx_of(k), j + cy_of(k))) continue;
+            const C v = get<S, C>(L, P, L.a, k, i, j);
+            fx += (double)((C)2.0 * (C)cx_of(k) * v);
+            fy += (double)((C)2.0 * (C)cy_of(k) * v);
+        }
+    }
+    sx[threadIdx.x] = fx;
+    sy[threadIdx.x] = fy;
+    __syncthreads();
+    for (int h = DL_T / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            sx[threadIdx.x] += sx[threadIdx.x + h];
+            sy[threadIdx.x] += sy[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sx[0];
+        out[1] = sy[0];
+    }
+}
+
 template <class S>
 __global__ void k_pack(Lat L, const void *plane, S *dense)
 {

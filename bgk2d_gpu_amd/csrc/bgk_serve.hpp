@@ -8,7 +8,8 @@
 // per-node code of k_fused (fused_node), so a member's populations are
 // bit-identical whether it runs alone or in any mix.  Each launch takes the
 // index t of the step inside a chunk; a member whose remaining step count is
-// <= t sits the launch out, and its FUSED parity is (parity0 + t) & 1.  The
+// <= t sits the launch out, and its FUSED parity is (parity0 + t) & 1 (the
+// same for the two pop-19 planes from rparity0).  The
 // host rewrites the small member/tile tables between chunks (admission and
 // retirement), never during one, so chunks replay from HIP graphs.
 #pragma once
@@ -25,13 +26,14 @@ enum MemberKind {
 
 struct Member {
     void *set[2][8];  // the two population sets
-    void *r;          // rest population
+    void *r[2];       // the two pop-19 planes
     int l, m, off, store;
     long long rs;
     int bc;                  // BcKind
     int tiles_x;             // tiles per row of tiles
     int tw_log2, th;         // tile shape: (1 << tw_log2) x th = 256 nodes
     int parity0, remaining;  // set holding the state at t = 0; steps left
+    int rparity0;            // pop-19 plane holding the state at t = 0
     Phys<float> P;
 };
 
@@ -49,6 +51,7 @@ __device__ __forceinline__ void member_bc(const Lat &L, const Phys<float> &P, in
     case BK_CHANNEL: member_node<S, BK_CHANNEL>(L, P, i, j); break;
     case BK_INFLOW: member_node<S, BK_INFLOW>(L, P, i, j); break;
     case BK_COUETTE: member_node<S, BK_COUETTE>(L, P, i, j); break;
+    case BK_OPEN: member_node<S, BK_OPEN>(L, P, i, j); break;
     default: member_node<S, BK_DRIVEN>(L, P, i, j); break;
     }
 }
@@ -73,7 +76,9 @@ __global__ __launch_bounds__(256) void k_batch(const Member *__restrict__ M, con
             L.a[k] = d.set[par][k];
             L.b[k] = d.set[par ^ 1][k];
         }
-        L.r = d.r;
+        const int rpar = (d.rparity0 + t) & 1;
+        L.r = d.r[rpar];
+        L.rn = d.r[rpar ^ 1];
         L.obs = nullptr;
         L.l = d.l;
         L.m = d.m;

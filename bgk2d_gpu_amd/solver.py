@@ -11,7 +11,8 @@ import numpy as np
 
 from . import build as _build
 
-BC = {"driven": 0, "periodic": 1, "channel": 2, "inflow": 3, "couette": 4, "step": 5}
+BC = {"driven": 0, "periodic": 1, "channel": 2, "inflow": 3, "couette": 4, "step": 5,
+      "open": 6}
 PREC = {"single": 0, "double": 1, "fp16": 2, "mixed2": 3, "bf16": 4}
 IMPL = {"original": 0, "fused": 1}
 STORE_DTYPE = {"single": np.float32, "double": np.float64, "fp16": np.uint16,
@@ -23,7 +24,8 @@ ERRORS = {-1: "invalid argument", -2: "out of memory", -3: "HIP error", -4: "div
 EXPORTS = ("bgk_create", "bgk_destroy", "bgk_last_error", "bgk_version", "bgk_constants",
            "bgk_build_obs", "bgk_set_obstacle", "bgk_get_obstacle", "bgk_init",
            "bgk_boundaries", "bgk_propagation", "bgk_collision", "bgk_run", "bgk_sync",
-           "bgk_diagno", "bgk_varm", "bgk_prof_i", "bgk_prof_j", "bgk_probe", "bgk_check",
+           "bgk_diagno", "bgk_varm", "bgk_prof_i", "bgk_prof_j", "bgk_probe", "bgk_draglift",
+           "bgk_check",
            "bgk_download", "bgk_upload", "bgk_storage_bytes", "bgk_stream", "bgk_last_run_ms",
            "bgk_batch_create", "bgk_batch_destroy", "bgk_batch_last_error", "bgk_footprint",
            "bgk_batch_add", "bgk_batch_ctx", "bgk_batch_remaining", "bgk_batch_remove",
@@ -78,6 +80,7 @@ def load(path=None):
         "bgk_prof_i": ([V, I32, P(D), P(D), P(D)], C.c_int),
         "bgk_prof_j": ([V, I32, P(D), P(D), P(D)], C.c_int),
         "bgk_probe": ([V, I32, I32, P(D)], C.c_int),
+        "bgk_draglift": ([V, I32, I32, I32, I32, P(D)], C.c_int),
         "bgk_check": ([V, D], C.c_int),
         "bgk_download": ([V, I32, V], C.c_int),
         "bgk_upload": ([V, I32, V], C.c_int),
@@ -225,6 +228,12 @@ class Solver:
         assert arr.shape == (9,) + self.plane_shape
         for k in range(9):
             self._chk(self.L.bgk_upload(self.h, k, arr[k].ctypes.data))
+
+    def draglift(self, box):
+        """(Fx, Fy) on the obstacle from the fluid nodes of box = (i0, i1, j0, j1)"""
+        out = np.zeros(2)
+        self._chk(self.L.bgk_draglift(self.h, *(int(b) for b in box), _dp(out)))
+        return out
 
     def obstacle(self):
         out = np.zeros((self.ly, self.lx), dtype=np.int32)

@@ -48,7 +48,12 @@ enum bgk_bc {
     BGK_BC_CHANNEL = 2,  /* bcond_channel  SRC/bcond.channel.F90:72-123  */
     BGK_BC_INFLOW = 3,   /* bcond_inflow   SRC/bcond.inflow.F90:59-153   */
     BGK_BC_COUETTE = 4,  /* bcond_couette  SRC/bcond.couette.F90:41-103  */
-    BGK_BC_STEP = 5      /* bcond_step     SRC/bcond.step.F90:55-254     */
+    BGK_BC_STEP = 5,     /* bcond_step     SRC/bcond.step.F90:55-254     */
+    BGK_BC_OPEN = 6      /* no reference counterpart: channel with a
+                            parabolic inflow of peak u0 at x = 0 (shifted
+                            equilibrium at the density of the first fluid
+                            node), zero-gradient outflow at x = l+1 and
+                            free-slip (specular) walls at y = 0, m+1     */
 };
 
 /* precision: SRC/precision.F90:13-47 plus bf16 storage */
@@ -148,6 +153,12 @@ int bgk_varm(bgk_ctx_t *ctx, double *u, double *v, double *rho);           /* m 
 int bgk_prof_i(bgk_ctx_t *ctx, int32_t jcoord, double *u, double *v, double *rho); /* l each */
 int bgk_prof_j(bgk_ctx_t *ctx, int32_t icoord, double *u, double *v, double *rho); /* m each */
 int bgk_probe(bgk_ctx_t *ctx, int32_t i0, int32_t j0, double out[3]);
+/* draglift (SRC/draglift.F90:58-98): momentum-exchange force on the
+ * obstacle, out = { Fx, Fy } = sum over the fluid nodes of the box
+ * [i0,i1] x [j0,j1] (clipped to the interior) of 2 c_k a_k for each
+ * population whose neighbour is solid, in FP64.  The reference's box is
+ * the cylinder centre +- (radius + 3); Cd = Fx / (U^2 radius).            */
+int bgk_draglift(bgk_ctx_t *ctx, int32_t i0, int32_t i1, int32_t j0, int32_t j1, double out[2]);
 /* 0 if every node is finite with |u| <= umax, else BGK_E_DIVERGED */
 int bgk_check(bgk_ctx_t *ctx, double umax);
 

@@ -31,6 +31,7 @@ struct orc {
     int imin, imax, jmin, jmax, nobs;
     int stepx, stepy;
     double omega, fgrad, u00, force;
+    double umax;  /* OPEN inflow peak in the compute kind */
     double scale, inv_scale;
     const orc_vtab *vt;
 };
@@ -47,6 +48,7 @@ struct orc_vtab {
     void (*prof_j)(const orc *, int, double *, double *, double *);
     void (*probe)(const orc *, int, int, double *);
     void (*decode_plane)(const orc *, const void *, double *);
+    void (*draglift)(const orc *, int, int, int, int, double *);
 };
 
 /* ---------------------------------------------------- storage rounding */
@@ -339,6 +341,31 @@ void orc_get_obs(const orc *h, int32_t *out, int32_t bbox[4])
 }
 
 void orc_diagno(const orc *h, int32_t mode, double out[4]) { h->vt->diagno(h, mode, out); }
+
+void orc_set_obs(orc *h, const int32_t *mask)
+{
+    const int l = h->l, m = h->m;
+    if (!h->obs) h->obs = (int32_t *)calloc((size_t)l * m, sizeof(int32_t));
+    h->imin = l; h->jmin = m; h->imax = 0; h->jmax = 0; h->nobs = 0;
+    for (int j = 1; j <= m; j++)
+        for (int i = 1; i <= l; i++) {
+            size_t q = (size_t)(i - 1) + (size_t)(j - 1) * l;
+            h->obs[q] = mask[q] == 1 ? 1 : 0;
+            if (h->obs[q]) {
+                h->nobs++;
+                if (i < h->imin) h->imin = i;
+                if (i > h->imax) h->imax = i;
+                if (j < h->jmin) h->jmin = j;
+                if (j > h->jmax) h->jmax = j;
+            }
+        }
+    h->p.obstacle = h->nobs > 0;
+}
+
+void orc_draglift(const orc *h, int32_t i0, int32_t i1, int32_t j0, int32_t j1, double out[2])
+{
+    h->vt->draglift(h, i0 < 1 ? 1 : i0, i1 > h->l ? h->l : i1, j0 < 1 ? 1 : j0, j1 > h->m ? h->m : j1, out);
+}
 
 void orc_varm(const orc *h, double *u, double *v, double *rho) { h->vt->varm(h, u, v, rho); }
 

@@ -27,7 +27,9 @@ extern "C" {
 #endif
 
 enum { ORC_BC_DRIVEN = 0, ORC_BC_PERIODIC = 1, ORC_BC_CHANNEL = 2,
-       ORC_BC_INFLOW = 3, ORC_BC_COUETTE = 4, ORC_BC_STEP = 5 };
+       ORC_BC_INFLOW = 3, ORC_BC_COUETTE = 4, ORC_BC_STEP = 5,
+       ORC_BC_OPEN = 6 /* parabolic inflow of peak u0, zero-gradient outflow,
+                          free-slip walls: no reference counterpart      */ };
 enum { ORC_SINGLE = 0, ORC_DOUBLE = 1, ORC_MIXED1 = 2, ORC_MIXED2 = 3,
        ORC_BF16 = 4 };
 enum { ORC_ORIGINAL = 0, ORC_FUSED = 1 };
@@ -70,6 +72,11 @@ void   orc_set_raw(orc *h, int32_t k, const void *in);
 /* raw storage bytes of the `b` set (FUSED: the next-step target)        */
 void   orc_get_raw_b(const orc *h, int32_t k, void *out);
 void   orc_get_obs(const orc *h, int32_t *out, int32_t bbox[4]);
+/* caller mask obs(1:l,1:m), x fastest (1 = solid); turns bcond_obs on   */
+void   orc_set_obs(orc *h, const int32_t *mask);
+/* draglift (SRC/draglift.F90:58-98) over the box, out = {Fx, Fy}: per-
+ * link terms 2 c_k a_k in compute kind, summed in long double           */
+void   orc_draglift(const orc *h, int32_t i0, int32_t i1, int32_t j0, int32_t j1, double out[2]);
 /* diagno: <u>,<v>,<rho>,<u2+v2>; per-node values in compute kind,
  * sums in long double (mode 0) or serially in compute kind exactly as
  * SRC/diagno.F90:63-101 (mode 1)                                        */

@@ -12,7 +12,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-BC = {"driven": 0, "periodic": 1, "channel": 2, "inflow": 3, "couette": 4, "step": 5}
+BC = {"driven": 0, "periodic": 1, "channel": 2, "inflow": 3, "couette": 4, "step": 5,
+      "open": 6}
 PREC = {"single": 0, "double": 1, "mixed1": 2, "mixed2": 3, "bf16": 4}
 IMPL = {"original": 0, "fused": 1}
 # reference build case -> (bc, obstacle)
@@ -61,6 +62,8 @@ def lib():
         L.orc_get_raw_b.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
         L.orc_set_raw.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
         L.orc_get_obs.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_int32)]
+        L.orc_set_obs.argtypes = [C.c_void_p, P(C.c_int32)]
+        L.orc_draglift.argtypes = [C.c_void_p] + [C.c_int32] * 4 + [P(C.c_double)]
         L.orc_diagno.argtypes = [C.c_void_p, C.c_int32, P(C.c_double)]
         L.orc_varm.argtypes = [C.c_void_p] + [P(C.c_double)] * 3
         L.orc_prof_i.argtypes = [C.c_void_p, C.c_int32] + [P(C.c_double)] * 3
@@ -154,6 +157,16 @@ class Oracle:
         lib().orc_get_obs(self.h, out.ctypes.data_as(C.POINTER(C.c_int32)),
                           bbox.ctypes.data_as(C.POINTER(C.c_int32)))
         return out, bbox
+
+    def set_obs(self, mask):
+        mask = np.ascontiguousarray(mask, dtype=np.int32)
+        assert mask.shape == (self.m, self.l)
+        lib().orc_set_obs(self.h, mask.ctypes.data_as(C.POINTER(C.c_int32)))
+
+    def draglift(self, box):
+        out = np.zeros(2)
+        lib().orc_draglift(self.h, *(int(b) for b in box), _dp(out))
+        return out
 
     def diagno(self, mode=0):
         out = np.zeros(4)

@@ -83,6 +83,8 @@ CASES = [
     ("couette", 0, 70, 45, 0.08, 0.05, 40),
     ("step", 0, 120, 40, 0.1, 0.1, 30),
     ("driven", 1, 90, 66, 0.05, 0.1, 30),
+    ("open", 1, 160, 64, 0.04, 0.05, 40),
+    ("open", 0, 97, 45, 0.06, 0.05, 40),
 ]
 
 
@@ -163,6 +165,7 @@ TB_CASES = [
     ("inflow", 1, 700, 200),
     ("couette", 0, 300, 97),
     ("driven", 1, 257, 513),
+    ("open", 1, 700, 200),
 ]
 
 

@@ -349,3 +349,93 @@ def test_pressure_consistent_taylor_green_matches_analytic():
     err = np.sqrt(((ux - ua) ** 2).sum() / (ua ** 2).sum())
     # O(k^2) lattice error of the decay rate at L = 48 (exponent 0.34)
     assert err < 4e-3
+
+
+# ------------------------------------------------ open channel, drag/lift
+
+@pytest.mark.parametrize("prec", ["single", "double"])
+@pytest.mark.parametrize("impl", ["fused", "original"])
+def test_open_free_slip_equals_mirrored_periodic(prec, impl):
+    """A free-slip (specular) wall at y = 1/2 is the mirror plane of a
+    periodic domain twice as high holding the field and its mirror image
+    (v -> -v: 01<->03, 08<->17, 10<->12).  One step of OPEN on (l, m) and of
+    the reference-pinned PERIODIC variant on (l, 2m) agree bit for bit on
+    every row, away from the columns the x boundaries touch."""
+    l, m = 30, 12
+    a = po.Oracle(l, m, bc="open", impl=impl, prec=prec, svisc=0.05, u0=0.04)
+    f = _random_state(a, amp=0.03, seed=5)
+    mir = np.array([1, 0, 2, 7, 5, 4, 6, 3, 8])
+    g = np.zeros((9, 2 * m + 2, l + 2))
+    g[:, 1:m + 1] = f[:, 1:m + 1]
+    g[:, m + 1:2 * m + 1] = f[mir][:, m:0:-1]  # row m+r <- mirror of row m+1-r
+    b = po.Oracle(l, 2 * m, bc="periodic", impl=impl, prec=prec, svisc=0.05)
+    b.set_raw(g)
+    a.set_raw(f)  # rounded to the storage kind the same way as g
+    a.step(1)
+    b.step(1)
+    ra, rb = a.raw(), b.raw()
+    assert np.array_equal(ra[:, 1:m + 1, 2:l], rb[:, 1:m + 1, 2:l])
+
+
+def test_open_channel_steady_flux_and_profile():
+    """Empty OPEN channel at steady state: the inlet column carries the
+    parabola 4 umax y'(1-y') (to 1.5 %, away from the wall rows, where the
+    free-slip flow does not vanish); the same mass flux through every
+    column; zero x-gradient at the outflow; a profile symmetric about the
+    centre line; and, the walls being free-slip (no shear), a nearly flat
+    profile downstream with the wall rows moving like the centre (a no-slip
+    wall would hold them back)."""
+    l, m, u0 = 120, 24, 0.05
+    o = po.Oracle(l, m, bc="open", impl="fused", prec="double", svisc=0.1, u0=u0)
+    o.step(10000)
+    f = o.pops()[:, 1:-1, 1:-1]
+    rho, ux, uy = moments(f)
+    y = (np.arange(1, m + 1) - 0.5) / m
+    np.testing.assert_allclose(ux[2:-2, 0], (4 * u0 * y * (1 - y))[2:-2], rtol=1.5e-2)
+    flux = (rho * ux).sum(0)
+    np.testing.assert_allclose(flux, flux.mean(), rtol=1e-9)
+    np.testing.assert_allclose(ux[:, -1], ux[:, -2], atol=1e-3 * u0)
+    np.testing.assert_allclose(ux, ux[::-1], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(uy, -uy[::-1], rtol=0, atol=1e-12)
+    assert np.argmax(ux[:, 0]) in (m // 2 - 1, m // 2) and np.all(np.diff(ux[:m // 2, 0]) > 0)
+    mid = ux[:, l // 2]
+    assert mid.std() < 0.02 * mid.mean()
+    assert ux[0, l // 2] > 0.95 * ux[m // 2, l // 2]
+    o.step(500)
+    f2 = o.pops()[:, 1:-1, 1:-1]
+    assert np.abs(f2 - f).max() < 1e-9  # steady
+
+
+def test_draglift_square_block_closed_form():
+    """Fluid nodes next to an s x s solid block, with a_k the same at every
+    node: each axis direction has s links into the block, each diagonal
+    2s - 1, so Fx = 2[(a01+a03-a10-a12)(2s-1) + (a05-a14) s] and
+    Fy = 2[(a03+a12-a01-a10)(2s-1) + (a08-a17) s]."""
+    l, m, s, x0, y0 = 20, 16, 4, 7, 6
+    o = po.Oracle(l, m, bc="periodic", impl="fused", prec="double", svisc=0.1)
+    mask = np.zeros((m, l), dtype=np.int32)
+    mask[y0 - 1:y0 - 1 + s, x0 - 1:x0 - 1 + s] = 1
+    o.set_obs(mask)
+    a = np.array([0.3, -0.7, 1.1, 0.45, -0.2, 0.9, -1.3, 0.05, 0.6])
+    o.set_raw(np.broadcast_to(a[:, None, None], (9, m + 2, l + 2)))
+    fx, fy = o.draglift((1, l, 1, m))
+    nd, na = 2 * s - 1, s
+    a01, a03, a05, a08, a10, a12, a14, a17 = a[:8]
+    assert fx == pytest.approx(2 * ((a01 + a03 - a10 - a12) * nd + (a05 - a14) * na), rel=1e-14)
+    assert fy == pytest.approx(2 * ((a03 + a12 - a01 - a10) * nd + (a08 - a17) * na), rel=1e-14)
+    # a box that misses the block's neighbours sees no force
+    assert np.array_equal(o.draglift((15, 20, 1, 16)), [0.0, 0.0])
+
+
+def test_open_symmetric_cylinder_has_no_lift():
+    """a cylinder centred on the channel's mirror line in the symmetric
+    OPEN inflow: the lift vanishes to round-off, the drag does not"""
+    l, m, R = 80, 40, 5
+    o = po.Oracle(l, m, bc="open", impl="fused", prec="double", svisc=0.05, u0=0.05)
+    jj, ii = np.mgrid[1:m + 1, 1:l + 1]
+    mask = ((ii - 25) ** 2 + (jj - (m + 1) / 2) ** 2 <= R * R).astype(np.int32)
+    o.set_obs(mask)
+    o.step(800)
+    fx, fy = o.draglift((25 - R - 3, 25 + R + 3, 1, m))
+    assert fx > 0
+    assert abs(fy) < 1e-9 * fx
