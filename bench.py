@@ -163,6 +163,44 @@ def serving_leg(n=48, seed=20261203, side=(128, 2048), steps=(1000, 10000), chun
             "tiles": tiles, "same_outcomes": same, "same_diagnostics": diag_same}
 
 
+def c4_leg(steps=2000):
+    """Channel past a cylinder: 8192 x 2048 OPEN channel (parabolic inflow
+    of peak 0.05, zero-gradient outflow, free-slip walls), cylinder of
+    diameter 128 centred at x = 1024 and one node above mid-height,
+    Re = u D / nu = 200 (nu = 0.032, tau = 0.596); FP32 and FP16 storage
+    (scale 1024, deviation form, stochastic rounding); MLUPS over `steps`
+    steps and the drag and lift coefficients Fx,y / (u^2 R) at the end."""
+    import numpy as np
+    from bgk2d_gpu_amd import Solver
+    l, m, D, u = 8192, 2048, 128, 0.05
+    nu = u * D / 200.0
+    xc, yc, r = 1024, m // 2 + 1, D // 2
+    jj, ii = np.mgrid[1:m + 1, 1:l + 1]
+    mask = ((ii - xc) ** 2 + (jj - yc) ** 2 <= r * r).astype(np.int32)
+    del ii, jj
+    box = (xc - r - 3, xc + r + 3, yc - r - 3, yc + r + 3)
+    out = {"workload": f"OPEN channel {l}x{m}, cylinder D={D} at ({xc},{yc}), u={u}, Re=200, "
+                       f"{steps} steps", "solid_nodes": int(mask.sum())}
+    for prec in ("single", "fp16"):
+        narrow = prec == "fp16"
+        s = Solver(l, m, bc="open", prec=prec, svisc=nu, u0=u, scale=1024.0 if narrow else 1.0,
+                   form=1 if narrow else 0, rounding=1 if narrow else 0)
+        s.set_obstacle(mask)
+        s.run(50)
+        s.sync()
+        t0 = time.perf_counter()
+        s.run(steps)
+        s.sync()
+        wall = time.perf_counter() - t0
+        fx, fy = s.draglift(box)
+        out[prec] = {"mlups": round(l * m * steps / wall / 1e6, 1),
+                     "kernel_ms_per_step": round(s.last_run_ms(), 5),
+                     "cd": round(fx / (u * u * r), 4), "cl": round(fy / (u * u * r), 4),
+                     "healthy": bool(s.check(0.5))}
+        s.close()
+    return out
+
+
 def copy_calib():
     """Plain device copies of a known size, for scaling the FETCH_SIZE /
     WRITE_SIZE counters to bytes."""
@@ -199,6 +237,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--serve-n", type=int, default=48)
+    ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--c4-only", action="store_true")
+    ap.add_argument("--c4-steps", type=int, default=2000)
     ap.add_argument("--precision-study", action="store_true",
                     help="only run the FP32/FP16/BF16 vs FP64 Taylor-Green error study")
     ap.add_argument("--study-n", type=int, default=8192)
@@ -219,6 +260,9 @@ def main():
         sys.exit(2)
     if args.serve_only:
         print(json.dumps(serving_leg(args.serve_n)))
+        return
+    if args.c4_only:
+        print(json.dumps(c4_leg(args.c4_steps)))
         return
 
     n = 4096
@@ -300,6 +344,8 @@ def main():
                                  "mlups_fp16": round(fp[("fp16", 2)][0], 1),
                                  "mlups_fp32": round(fp[("single", 2)][0], 1),
                                  "ratio": round(fp[("fp16", 2)][0] / fp[("single", 2)][0], 3)}}
+    if not args.no_c4:
+        out["cylinder_channel"] = c4_leg(args.c4_steps)
     if not args.no_serve:
         out["serving"] = serving_leg(args.serve_n)
     if not args.no_cpu:

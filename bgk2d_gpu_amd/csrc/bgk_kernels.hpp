@@ -448,14 +448,19 @@ __device__ C folded_a(const A &a, const Phys<C> &P, int l, int m, int k, int i, 
 {
     if (BC == BK_PERIODIC) return a(k, si, sj);
     if (BC == BK_OPEN) {
-        int kk = k, rj = sj;
-        if (sj < 1 || sj > m) {  // the y pass copies the mirror population of row 1 / m
-            kk = mirror_y(k);
-            rj = sj < 1 ? 1 : m;
+        if (sj < 1 || sj > m) {
+            // the y pass copies the mirror population of row 1 / m, which
+            // is the node's own row (sj < 1 only for j = 1, sj > m only for
+            // j = m); kept apart from the x rule below so that both
+            // population indices stay compile-time constants
+            const int kk = mirror_y(k);
+            if (si < 1) return rt<S, C>(P, open_in<C>(P, kk, j, m, drho_of<C>(a, 1, j)), 0, j, kk);
+            if (si > l) return a(kk, l, j);
+            return a(kk, si, j);
         }
-        if (si < 1) return rt<S, C>(P, open_in<C>(P, kk, rj, m, drho_of<C>(a, 1, rj)), 0, rj, kk);
-        if (si > l) return a(kk, l, rj);
-        return a(kk, si, rj);
+        if (si < 1) return rt<S, C>(P, open_in<C>(P, k, sj, m, drho_of<C>(a, 1, sj)), 0, sj, k);
+        if (si > l) return a(k, l, sj);
+        return a(k, si, sj);
     }
     if (BC == BK_INFLOW) {
         const int sjw = sj < 1 ? m : (sj > m ? 1 : sj);  // the y pass runs after the x pass
@@ -506,8 +511,8 @@ __device__ C folded(const Lat &L, const Phys<C> &P, int k, int i, int j, int si,
 // wall boundary before the obstacle one.  With WRAPPED the caller's pulls
 // already hold the wrapped values along periodic directions, so only the
 // sources outside a wall-bounded direction are replaced.
-template <class S, class C, int BC, bool OBS, bool WRAPPED = false, class A>
-__device__ __forceinline__ void fix_pulls_a(const A &a, const Lat &L, const Phys<C> &P, int i, int j, C x[9])
+template <class S, class C, int BC, bool OBS, bool WRAPPED, class A>
+__device__ __forceinline__ void fix_body(const A &a, const Lat &L, const Phys<C> &P, int i, int j, C *x)
 {
     constexpr bool fx = !(WRAPPED && Per<BC>::x), fy = !(WRAPPED && Per<BC>::y);
     if ((fx && (i == 1 || i == L.l)) || (fy && (j == 1 || j == L.m))) {
@@ -528,6 +533,15 @@ __device__ __forceinline__ void fix_pulls_a(const A &a, const Lat &L, const Phys
             if (solid(L, si, sj) && !(me && before_pop(k))) x[k] = a(opp_of(k), i, j);
         }
     }
+}
+
+template <class S, class C, int BC, bool OBS, bool WRAPPED = false, class A>
+__device__ __forceinline__ void fix_pulls_a(const A &a, const Lat &L, const Phys<C> &P, int i, int j, C x[9])
+{
+    constexpr bool fx = !(WRAPPED && Per<BC>::x), fy = !(WRAPPED && Per<BC>::y);
+    const bool edge = (fx && (i == 1 || i == L.l)) || (fy && (j == 1 || j == L.m));
+    const bool near = OBS && i >= L.imin - 1 && i <= L.imax + 1 && j >= L.jmin - 1 && j <= L.jmax + 1;
+    if (edge || near) fix_body<S, C, BC, OBS, WRAPPED>(a, L, P, i, j, x);
 }
 
 template <class S, class C, int BC, bool OBS>
@@ -589,6 +603,18 @@ __global__ __launch_bounds__(256) void k_fused2(Lat L, Phys<C> P)
     const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * 2 + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     if (i0 > L.l || j > L.m) return;
+    // With the column-l / inlet equilibria of INFLOW and OPEN, the boundary
+    // code inlined twice below would raise the register count of every pair
+    // (to ~150): there, pairs that touch an edge or the obstacle take the
+    // one-node path instead.  The other variants keep it inline, which is
+    // faster for them (the obstacle rule included).
+    constexpr bool split = BC == BK_INFLOW || BC == BK_OPEN;
+    if (split && (i0 == 1 || i0 + 1 >= L.l || j == 1 || j == L.m ||
+                  (OBS && i0 + 1 >= L.imin - 1 && i0 <= L.imax + 1 && j >= L.jmin - 1 && j <= L.jmax + 1))) {
+        fused_node<S, C, BC, OBS>(L, P, i0, j);
+        if (i0 + 1 <= L.l) fused_node<S, C, BC, OBS>(L, P, i0 + 1, j);
+        return;
+    }
     using R = typename CodecT<S, C>::raw;
     const uint32_t bo = ((uint32_t)j * (uint32_t)L.rs + (uint32_t)i0) * 2u;
     C x[2][9], o[9];
@@ -611,7 +637,7 @@ __global__ __launch_bounds__(256) void k_fused2(Lat L, Phys<C> P)
 #pragma unroll
     for (int q = 0; q < 2; q++) {
         const int i = i0 + q;
-        fix_pulls<S, C, BC, OBS>(L, P, i, j, x[q]);
+        if (!split) fix_pulls<S, C, BC, OBS>(L, P, i, j, x[q]);
         relax<C>(P, x[q], o);
 #pragma unroll
         for (int k = 0; k < 9; k++) {
