@@ -9,7 +9,10 @@
 ! formats.  The reference's compile-time choices (-DLDC/-DTGV/..., SINGLE/
 ! DOUBLE/MIXED1/MIXED2, ORIG/FUSED) are read at run time from an optional
 ! second namelist &gpu in the same file:
-!     case_name      'LDC' 'TGV' 'POF' 'VKS' 'COU' 'BFS'   (default 'LDC')
+!     case_name      'LDC' 'TGV' 'POF' 'VKS' 'COU' 'BFS', or 'CYL': the
+!                    open channel past a cylinder (parabolic inflow of
+!                    peak u0, zero-gradient outflow, free-slip walls)
+!                    (default 'LDC')
 !     precision      'SINGLE' 'DOUBLE' 'MIXED1' 'MIXED2' 'FP16' 'BF16'
 !     implementation 'ORIGINAL' 'FUSED'                    (default 'FUSED')
 !     scale          FP16/BF16 storage scale, a power of two (default 1)
@@ -19,6 +22,11 @@
 !     device         HIP device ordinal
 !     form           0: reference operation order; 1: deviation form
 !     rounding       FP16/BF16 stores: 0 nearest even; 1 stochastic
+!     obs_x, obs_y   cylinder centre of VKS/CYL (default: the reference's
+!                    2l/5, m/2, SRC/build.obs.F90:174-175); radius from
+!                    &parameters
+!     drag           1: write drag.lift.dat at every icheck step, as the
+!                    DRAG build does (SRC/diagnostic.F90:80-82)
 program bgk2d_hip
   use iso_c_binding
   use bgk_hip
@@ -29,9 +37,9 @@ program bgk2d_hip
   real(8) :: svisc, u0, cteS
   character(len=16) :: case_name, precision, implementation
   real(8) :: scale, amplitude
-  integer :: phases, device, form, rounding
+  integer :: phases, device, form, rounding, obs_x, obs_y, drag
   namelist /gpu/ case_name, precision, implementation, scale, amplitude, phases, device, &
-       form, rounding
+       form, rounding, obs_x, obs_y, drag
 
   type(bgk_params_t) :: p
   type(c_ptr) :: ctx
@@ -48,6 +56,7 @@ program bgk2d_hip
   open(64, file='prof_j.dat', status='unknown')
   open(62, file='u_med.dat', status='unknown')
   open(63, file='diagno.dat', status='unknown')
+  open(66, file='drag.lift.dat', status='unknown')
 
   case_name = 'LDC'
   precision = 'SINGLE'
@@ -58,6 +67,9 @@ program bgk2d_hip
   device = 0
   form = 0
   rounding = 0
+  obs_x = -1
+  obs_y = -1
+  drag = 0
   open(15, file='bgk.input', status='old', iostat=ios)
   if (ios /= 0) error stop 'bgk.input not found'
   read(15, gpu, iostat=ios)
@@ -86,6 +98,7 @@ program bgk2d_hip
   case ('VKS'); p%bc = BGK_BC_INFLOW; p%obstacle = 1
   case ('COU'); p%bc = BGK_BC_COUETTE
   case ('BFS'); p%bc = BGK_BC_STEP
+  case ('CYL'); p%bc = BGK_BC_OPEN; p%obstacle = 1
   case default; error stop 'unknown case_name'
   end select
   select case (trim(precision))
@@ -114,7 +127,15 @@ program bgk2d_hip
        trim(implementation)
 
   ! -------------------------------------------------------- initialize
-  if (p%obstacle == 1) call bgk_ok(ctx, bgk_build_obs(ctx), 'bgk_build_obs')
+  if (p%obstacle == 1) then
+     if (obs_x > 0 .and. obs_y > 0) then
+        call set_cylinder()
+     else
+        call bgk_ok(ctx, bgk_build_obs(ctx), 'bgk_build_obs')
+        obs_x = 2 * l / 5
+        obs_y = m / 2
+     end if
+  end if
   if (irestart == 1) then
      call restore_raw(itstart)
      write(6,*) 'INFO: restoring at timestep ', itstart
@@ -235,9 +256,42 @@ contains
        call prof_j_out(it, l/2)
        rc = bgk_check(ctx, 0.5d0)
        if (rc /= 0) write(16,*) 'WARNING: non-finite or |u| > 0.5 at step ', it
+       if (drag == 1 .and. p%obstacle == 1) call draglift_out(it)
     end if
     if (mod(it, itsave) == 0) call save_raw(it)
   end subroutine diagnostic
+
+  ! cylinder (i-obs_x)^2 + (j-obs_y)^2 <= radius^2 as build_obs's
+  ! (SRC/build.obs.F90:170-205), at a chosen centre
+  subroutine set_cylinder()
+    integer(c_int32_t), allocatable :: mask(:)
+    integer :: i, j
+    allocate(mask(l * m))
+    do j = 1, m
+       do i = 1, l
+          mask(i + (j - 1) * l) = 0
+          if ((i - obs_x)**2 + (j - obs_y)**2 <= radius * radius) mask(i + (j - 1) * l) = 1
+       end do
+    end do
+    call bgk_ok(ctx, bgk_set_obstacle(ctx, mask), 'bgk_set_obstacle')
+    deallocate(mask)
+  end subroutine set_cylinder
+
+  ! draglift (SRC/draglift.F90:41-101): box = centre +- (radius + 3),
+  ! coefficients F / (U^2 radius), U the inflow speed (VKS: u_inflow = 0.1)
+  subroutine draglift_out(it)
+    integer, intent(in) :: it
+    real(8) :: f(2), uin, norm
+    integer, parameter :: delta = 3
+    uin = 0.1d0
+    if (p%bc == BGK_BC_OPEN) uin = u0
+    norm = 1.0d0 / (uin * uin * real(radius, 8))
+    call bgk_ok(ctx, bgk_draglift(ctx, int(obs_x - radius - delta, c_int32_t), &
+         int(obs_x + radius + delta, c_int32_t), int(obs_y - radius - delta, c_int32_t), &
+         int(obs_y + radius + delta, c_int32_t), f), 'bgk_draglift')
+    write(66,*) it, f(1) * norm, f(2) * norm
+    flush(66)
+  end subroutine draglift_out
 
   subroutine diagno_out(it)
     integer, intent(in) :: it

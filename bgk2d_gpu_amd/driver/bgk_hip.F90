@@ -10,7 +10,8 @@ module bgk_hip
   implicit none
 
   integer(c_int32_t), parameter :: BGK_BC_DRIVEN = 0, BGK_BC_PERIODIC = 1, &
-       BGK_BC_CHANNEL = 2, BGK_BC_INFLOW = 3, BGK_BC_COUETTE = 4, BGK_BC_STEP = 5
+       BGK_BC_CHANNEL = 2, BGK_BC_INFLOW = 3, BGK_BC_COUETTE = 4, BGK_BC_STEP = 5, &
+       BGK_BC_OPEN = 6
   integer(c_int32_t), parameter :: BGK_PREC_SINGLE = 0, BGK_PREC_DOUBLE = 1, &
        BGK_PREC_FP16 = 2, BGK_PREC_MIXED2 = 3, BGK_PREC_BF16 = 4
   integer(c_int32_t), parameter :: BGK_IMPL_ORIGINAL = 0, BGK_IMPL_FUSED = 1
@@ -43,6 +44,17 @@ module bgk_hip
      integer(c_int) function bgk_build_obs(ctx) bind(c, name='bgk_build_obs')
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
+     end function
+     integer(c_int) function bgk_set_obstacle(ctx, obs) bind(c, name='bgk_set_obstacle')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(in) :: obs(*)
+     end function
+     integer(c_int) function bgk_draglift(ctx, i0, i1, j0, j1, out) bind(c, name='bgk_draglift')
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: i0, i1, j0, j1
+       real(c_double), intent(out) :: out(2)
      end function
      integer(c_int) function bgk_init(ctx, init_v, amplitude) bind(c, name='bgk_init')
        import :: c_int, c_ptr, c_int32_t, c_double

@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:9a9a0f25de17eb82
+﻿!mod$ v1 sum:d94afce11da7c647
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ bb381bf46e508468 i __fortran_builtins
 module bgk_hip
@@ -85,6 +85,7 @@ integer(4),parameter::bgk_bc_channel=2_4
 integer(4),parameter::bgk_bc_inflow=3_4
 integer(4),parameter::bgk_bc_couette=4_4
 integer(4),parameter::bgk_bc_step=5_4
+integer(4),parameter::bgk_bc_open=6_4
 integer(4),parameter::bgk_prec_single=0_4
 integer(4),parameter::bgk_prec_double=1_4
 integer(4),parameter::bgk_prec_fp16=2_4
@@ -106,7 +107,7 @@ integer(4)::device
 integer(4)::mixed1_quirk
 integer(4)::form
 integer(4)::rounding
-integer(4)::reserved
+integer(4)::tblock
 real(8)::svisc
 real(8)::u0
 real(8)::scale
@@ -147,6 +148,26 @@ function bgk_build_obs(ctx) bind(c,name="bgk_build_obs")
 import::c_ptr
 type(c_ptr),value::ctx
 integer(4)::bgk_build_obs
+end
+end interface
+interface
+function bgk_set_obstacle(ctx,obs) bind(c,name="bgk_set_obstacle")
+import::c_ptr
+type(c_ptr),value::ctx
+integer(4),intent(in)::obs(1_8:*)
+integer(4)::bgk_set_obstacle
+end
+end interface
+interface
+function bgk_draglift(ctx,i0,i1,j0,j1,out) bind(c,name="bgk_draglift")
+import::c_ptr
+type(c_ptr),value::ctx
+integer(4),value::i0
+integer(4),value::i1
+integer(4),value::j0
+integer(4),value::j1
+real(8),intent(out)::out(1_8:2_8)
+integer(4)::bgk_draglift
 end
 end interface
 interface

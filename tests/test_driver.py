@@ -21,19 +21,22 @@ itsave={itsave}
 icheck={icheck}
 irestart={irestart}
 init_v=0
-ipad=0        /
+ipad=0
+radius={radius}        /
 &gpu
 case_name='{case}'
 precision='{prec}'
 implementation='{impl}'
 phases={phases}
+{extra}
 /
 """
 
 
 def write_input(d, **kw):
     args = dict(lx=64, ly=48, svisc=0.05, u0=0.1, itfin=300, isignal=100, itsave=300,
-                icheck=100, irestart=0, case="LDC", prec="SINGLE", impl="FUSED", phases=0)
+                icheck=100, irestart=0, case="LDC", prec="SINGLE", impl="FUSED", phases=0,
+                radius=16, extra="")
     args.update(kw)
     with open(os.path.join(d, "bgk.input"), "w") as f:
         f.write(INPUT.format(**args))
@@ -114,3 +117,27 @@ def test_driver_restart_continues_identically(tmp_path):
     _, a = read_save(os.path.join(d1, "save.00000400.bin"), np.float32, 64, 48)
     _, b = read_save(os.path.join(d2, "save.00000400.bin"), np.float32, 64, 48)
     assert np.array_equal(a[:, 1:-1, 1:-1], b[:, 1:-1, 1:-1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("phases", [0, 1])
+def test_driver_cylinder_channel_and_draglift(tmp_path, phases):
+    """case CYL: open channel past a cylinder at a chosen centre; the saved
+    populations equal the oracle's and drag.lift.dat holds F / (u0^2 R)"""
+    d = str(tmp_path)
+    l, m, r, xc, yc, u0 = 120, 48, 6, 30, 25, 0.05
+    write_input(d, case="CYL", lx=l, ly=m, svisc=0.05, u0=u0, itfin=200, itsave=200,
+                icheck=100, radius=r, phases=phases, extra=f"obs_x={xc}\nobs_y={yc}\ndrag=1")
+    res = run_driver(d)
+    assert res.returncode == 0, res.stdout
+    o = po.Oracle(l, m, bc="open", impl="fused", prec="single", svisc=0.05, u0=u0)
+    jj, ii = np.mgrid[1:m + 1, 1:l + 1]
+    o.set_obs(((ii - xc) ** 2 + (jj - yc) ** 2 <= r * r).astype(np.int32))
+    o.step(200)
+    _, pops = read_save(os.path.join(d, "save.00000200.bin"), np.float32, l, m)
+    ref = o.raw()
+    assert np.array_equal(pops[:, 1:-1, 1:-1].view(np.uint32), ref[:, 1:-1, 1:-1].view(np.uint32))
+    dl = np.loadtxt(os.path.join(d, "drag.lift.dat"))
+    assert dl.shape == (2, 3) and dl[-1, 0] == 200
+    f = o.draglift((xc - r - 3, xc + r + 3, yc - r - 3, yc + r + 3)) / (u0 * u0 * r)
+    np.testing.assert_allclose(dl[-1, 1:], f, rtol=1e-7)
