@@ -130,6 +130,8 @@ Phys<C> phys(const bgk_ctx *c)
     P.umax = (C)c->umax;
     P.channel_force = c->p.bc == BGK_BC_CHANNEL ? 1 : 0;
     P.form = c->p.form;
+    P.les = c->p.les;
+    P.ctes = (C)c->p.cte_s;
     P.rounding = (is_fp16(c->p.prec) || is_bf16(c->p.prec)) ? c->p.rounding : 0;
     return P;
 }
@@ -617,6 +619,7 @@ int bgk_create(const bgk_params_t *p, bgk_ctx_t **out)
     if (p->form != 0 && p->form != 1) return BGK_E_INVALID;
     if (p->rounding != 0 && p->rounding != 1) return BGK_E_INVALID;
     if (p->tblock < 0 || p->tblock > 2) return BGK_E_INVALID;
+    if ((p->les != 0 && p->les != 1) || p->reserved != 0 || !(p->cte_s >= 0.0)) return BGK_E_INVALID;
     bgk_ctx *c = new bgk_ctx();
     c->p = *p;
     c->l = p->lx;

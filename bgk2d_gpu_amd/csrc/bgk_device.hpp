@@ -152,6 +152,8 @@ struct Phys {
     int channel_force;      // 1: CHANNEL body force in collision
     int form;               // 0: reference operation order; 1: deviation form
     int rounding;           // FP16/BF16 stores: 0 nearest-even, 1 stochastic
+    int les;                // 1: Smagorinsky LES relaxation rate per node
+    C ctes;                 // LES: Smagorinsky constant cteS
 };
 
 // ---------------------------------------------------------- arithmetic
@@ -209,9 +211,25 @@ __device__ __forceinline__ void relax(const Phys<C> &P, const C x[9], C o[9])
     C e14 = rp1 * (-vx + qx) + d1;
     C e17 = rp1 * (-vy + qy) + d1;
     C e19 = rp0 * (+q0) + d0;
+    C om = P.omega;
+    if (P.les) {
+        // Smagorinsky LES (SRC/col_MC.F90:178-230, SRC/col.F90:154-205):
+        // the relaxation rate from the non-equilibrium stress; omega1 is the
+        // molecular rate of hencol (= P.omega)
+        const C n01 = x01 - e01, n03 = x03 - e03, n05 = x05 - e05, n08 = x08 - e08;
+        const C n10 = x10 - e10, n12 = x12 - e12, n14 = x14 - e14, n17 = x17 - e17;
+        const C pxx = n01 + n03 + n05 + n10 + n12 + n14;
+        const C pyy = n01 + n03 + n08 + n10 + n12 + n17;
+        const C pxy = -n01 + n03 + n10 - n12;
+        const C pyx = pxy;
+        const C ptot = sqrt(pxx * pxx + (C)2.0 * pxy * pyx + pyy * pyy);
+        const C omega1 = P.omega, cs = P.ctes;
+        const C inv1 = (C)1.0 / omega1;
+        const C ts = (C)1.0 / ((C)2.0 * omega1) + sqrt((C)18.0 * (cs * cs) * ptot + inv1 * inv1) / (C)2.0;
+        om = (C)1.0 / ts;
+    }
     C forcex = (C)0.0, forcey = (C)0.0;
     if (P.channel_force) forcex = P.fgrad * rho;
-    const C om = P.omega;
     o[0] = x01 - om * (x01 - e01) + (+forcex - forcey);
     o[1] = x03 - om * (x03 - e03) + (+forcex + forcey);
     o[2] = x05 - om * (x05 - e05) + (+forcex);

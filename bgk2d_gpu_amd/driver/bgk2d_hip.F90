@@ -27,6 +27,8 @@
 !                    &parameters
 !     drag           1: write drag.lift.dat at every icheck step, as the
 !                    DRAG build does (SRC/diagnostic.F90:80-82)
+!     les            1: Smagorinsky LES collision, as the LES build does,
+!                    with &parameters cteS
 program bgk2d_hip
   use iso_c_binding
   use bgk_hip
@@ -37,9 +39,9 @@ program bgk2d_hip
   real(8) :: svisc, u0, cteS
   character(len=16) :: case_name, precision, implementation
   real(8) :: scale, amplitude
-  integer :: phases, device, form, rounding, obs_x, obs_y, drag
+  integer :: phases, device, form, rounding, obs_x, obs_y, drag, les
   namelist /gpu/ case_name, precision, implementation, scale, amplitude, phases, device, &
-       form, rounding, obs_x, obs_y, drag
+       form, rounding, obs_x, obs_y, drag, les
 
   type(bgk_params_t) :: p
   type(c_ptr) :: ctx
@@ -70,6 +72,7 @@ program bgk2d_hip
   obs_x = -1
   obs_y = -1
   drag = 0
+  les = 0
   open(15, file='bgk.input', status='old', iostat=ios)
   if (ios /= 0) error stop 'bgk.input not found'
   read(15, gpu, iostat=ios)
@@ -89,6 +92,8 @@ program bgk2d_hip
   p%form = form
   p%rounding = rounding
   p%tblock = 0
+  p%les = les
+  p%reserved = 0
   p%obstacle = 0
   p%scale = scale
   select case (trim(case_name))
@@ -118,6 +123,7 @@ program bgk2d_hip
   call read_parameters_kind()
   p%svisc = svisc
   p%u0 = u0
+  p%cte_s = cteS
 
   ctx = c_null_ptr
   call bgk_ok(ctx, bgk_create(p, ctx), 'bgk_create')
@@ -209,7 +215,7 @@ contains
     flag2 = 0
     flag3 = 0
     init_v = 0
-    cteS = 0.1d0
+    cteS = 0.1   ! a default-real literal, as SRC/input.F90:55
     ipad = 0
     jpad = 0
     radius = 16
@@ -222,25 +228,28 @@ contains
   ! kind, as the reference's namelist read into real(mykind) does; for an
   ! fp32 compute kind the single value is passed on exactly as a double.
   subroutine read_parameters_kind()
-    real(4) :: svisc4, u04
+    real(4) :: svisc4, u04, ctes4
     if (p%prec == BGK_PREC_DOUBLE .or. p%prec == BGK_PREC_MIXED2) return
-    call read_sp(svisc4, u04)
+    call read_sp(svisc4, u04, ctes4)
     svisc = real(svisc4, 8)
     u0 = real(u04, 8)
+    cteS = real(ctes4, 8)
   end subroutine read_parameters_kind
 
-  subroutine read_sp(svisc4, u04)
-    real(4), intent(out) :: svisc4, u04
+  subroutine read_sp(svisc4, u04, ctes4)
+    real(4), intent(out) :: svisc4, u04, ctes4
     real(4) :: svisc, u0, cteS
     integer :: itfin, ivtim, isignal, itsave, icheck, irestart, init_v, lx, ly
     integer :: flag1, flag2, flag3, ipad, jpad, radius
     namelist /parameters/ svisc, u0, itfin, ivtim, isignal, itsave, icheck, irestart, &
          init_v, lx, ly, flag1, flag2, flag3, ipad, jpad, radius, cteS
+    cteS = 0.1
     open(15, file='bgk.input', status='old')
     read(15, parameters)
     close(15)
     svisc4 = svisc
     u04 = u0
+    ctes4 = cteS
   end subroutine read_sp
 
   ! diagnostic(itime,ivtim,icheck,itsave) (SRC/diagnostic.F90:32-103);

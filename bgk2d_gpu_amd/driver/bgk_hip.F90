@@ -19,7 +19,8 @@ module bgk_hip
   type, bind(c) :: bgk_params_t
      integer(c_int32_t) :: lx, ly, ipad, jpad, bc, obstacle, impl, prec
      integer(c_int32_t) :: radius, device, mixed1_quirk, form, rounding, tblock
-     real(c_double)     :: svisc, u0, scale
+     integer(c_int32_t) :: les, reserved
+     real(c_double)     :: svisc, u0, scale, cte_s
   end type bgk_params_t
 
   interface

@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:d94afce11da7c647
+﻿!mod$ v1 sum:396cc9bfe76a6f00
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ bb381bf46e508468 i __fortran_builtins
 module bgk_hip
@@ -108,9 +108,12 @@ integer(4)::mixed1_quirk
 integer(4)::form
 integer(4)::rounding
 integer(4)::tblock
+integer(4)::les
+integer(4)::reserved
 real(8)::svisc
 real(8)::u0
 real(8)::scale
+real(8)::cte_s
 end type
 interface
 function bgk_create(p,ctx) bind(c,name="bgk_create")

@@ -58,9 +58,10 @@ class Request:
         run the same per-node code."""
         lx, ly, bc, svisc, u0, _ = self.geometry()
         narrow = self.prec in ("fp16", "bf16")
-        return BgkParams(lx, ly, 0, 0, BC[bc], 0, IMPL["fused"], PREC[self.prec], 16, device, 0,
-                         form if narrow else 0, rounding if narrow else 0, tblock, float(svisc),
-                         float(u0), 1024.0 if narrow else 1.0)
+        return BgkParams(lx=lx, ly=ly, bc=BC[bc], impl=IMPL["fused"], prec=PREC[self.prec],
+                         radius=16, device=device, form=form if narrow else 0,
+                         rounding=rounding if narrow else 0, tblock=tblock, svisc=float(svisc),
+                         u0=float(u0), scale=1024.0 if narrow else 1.0, cte_s=0.1)
 
 
 def make_stream(n, seed=20261203, rate=200.0, side=(128, 4096), re=(100.0, 1e4),

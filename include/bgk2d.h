@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define BGK_ABI_VERSION 1
+#define BGK_ABI_VERSION 2
 
 /* boundary variant: the compile-time -DLDC/-DPERIODIC/-DCHANNEL/-DINFLOW/
  * -DCOUETTE/-DSTEP choice of SRC/boundaries.F90:31-56, made at run time  */
@@ -98,9 +98,14 @@ typedef struct bgk_params {
                              library's choice (2 for fp32/fp64 storage, 1
                              for FP16/BF16), 1 one step per kernel, 2 two
                              steps per kernel; results are identical       */
+    int32_t les;          /* 1: Smagorinsky LES collision, the -DLES build
+                             (SRC/col_MC.F90:178-230)                       */
+    int32_t reserved;     /* must be 0                                      */
     double svisc;         /* namelist svisc (kinematic viscosity)           */
     double u0;            /* namelist u0 (lid speed, or channel force scale)*/
     double scale;         /* FP16/BF16 storage scale (power of two, >= 1)   */
+    double cte_s;         /* LES: namelist cteS (SRC/input.F90:55 defaults it
+                             to 0.1, a default-real literal)                */
 } bgk_params_t;
 
 typedef struct bgk_ctx bgk_ctx_t;

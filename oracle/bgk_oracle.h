@@ -46,9 +46,14 @@ typedef struct {
     int32_t tgv;           /* 1: Taylor-Green initial field (PERIODIC)    */
     int32_t form;          /* 0: reference operation order; 1: deviation  */
     int32_t rounding;      /* half/bf16 stores: 0 nearest, 1 stochastic   */
+    int32_t les;           /* 1: Smagorinsky LES collision (-DLES build)  */
+    int32_t reserved;
     double svisc, u0;      /* namelist values                             */
     double amplitude;      /* TGV amplitude (reference hard-codes 0.1)    */
     double scale;          /* storage scale for half/bf16 (power of two)  */
+    double cte_s;          /* Smagorinsky constant cteS, taken into the
+                              compute kind (reference default 0.1, a
+                              default-real literal)                       */
 } orc_params;
 
 typedef struct orc orc;

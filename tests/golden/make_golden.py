@@ -49,6 +49,12 @@ CASES = [
     ("MIXED1_FUSED_TGV@96x70", 96, 70, 0.02, 0.1, 3, 16),
     ("SINGLE_FUSED_LDC@130x66", 130, 66, 0.02, 0.1, 20, 16),
     ("SINGLE_FUSED_VKS@100x60", 100, 60, 0.04, 0.1, 30, 8),
+    # Smagorinsky LES (-DLES, cteS = 0.1) at low viscosity
+    ("SINGLE_FUSED_LDCLES", 24, 20, 0.005, 0.1, 60, 16),
+    ("SINGLE_ORIG_LDCLES", 24, 20, 0.005, 0.1, 60, 16),
+    ("DOUBLE_FUSED_LDCLES", 24, 20, 0.005, 0.1, 60, 16),
+    ("MIXED2_FUSED_LDCLES", 24, 20, 0.005, 0.1, 60, 16),
+    ("SINGLE_FUSED_VKSLES", 40, 24, 0.005, 0.1, 60, 4),
 ]
 
 

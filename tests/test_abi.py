@@ -24,12 +24,12 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(pkg.build.LIB)
     for n in declared():
         assert hasattr(lib, n), n
-    assert lib.bgk_version() == 1
+    assert lib.bgk_version() == 2
 
 
 def test_struct_layout_matches_header():
     # bgk_params_t: 14 int32 then 3 doubles, no padding
-    assert ctypes.sizeof(pkg.solver.BgkParams) == 14 * 4 + 3 * 8
+    assert ctypes.sizeof(pkg.solver.BgkParams) == 16 * 4 + 4 * 8
 
 
 def test_null_arguments_rejected_without_gpu():

@@ -439,3 +439,27 @@ def test_open_symmetric_cylinder_has_no_lift():
     fx, fy = o.draglift((25 - R - 3, 25 + R + 3, 1, m))
     assert fx > 0
     assert abs(fy) < 1e-9 * fx
+
+
+def test_les_without_smagorinsky_term_is_bgk():
+    """cteS = 0 leaves Ts = 1/(2 omega) + (1/omega)/2 = 1/omega: the LES
+    collision reduces to plain BGK (to round-off of that identity)"""
+    kw = dict(bc="driven", impl="fused", prec="double", svisc=0.02, u0=0.1)
+    a = po.Oracle(40, 30, **kw)
+    b = po.Oracle(40, 30, les=1, cte_s=0.0, **kw)
+    a.step(200)
+    b.step(200)
+    np.testing.assert_allclose(b.pops(), a.pops(), rtol=0, atol=1e-13)
+
+
+def test_les_only_adds_dissipation():
+    """the Smagorinsky viscosity is non-negative: a Taylor-Green vortex
+    loses kinetic energy at least as fast with LES, and faster the larger
+    cteS"""
+    kw = dict(bc="periodic", impl="fused", prec="double", svisc=0.002, amplitude=0.05)
+    e = []
+    for les, cs in ((0, 0.0), (1, 0.1), (1, 0.3)):
+        o = po.Oracle(48, 48, les=les, cte_s=cs, **kw)
+        o.step(400)
+        e.append(o.diagno()[3])
+    assert e[0] > e[1] > e[2]
