@@ -38,6 +38,7 @@ struct bgk_ctx {
     std::vector<int32_t> h_obs;
     double *d_rows, *d_red, *h_red;
     void *d_stage;
+    float *d_fields;  // bgk_fields output, allocated on first use
     size_t stage_bytes;
     // constants in the compute kind, widened
     double omega, fgrad, u00, force, scale;
@@ -522,6 +523,17 @@ int draglift_t(bgk_ctx *c, int i0, int i1, int j0, int j1)
     return BGK_OK;
 }
 
+template <class S, class C>
+int fields_t(bgk_ctx *c, float *d)
+{
+    Lat L = lat(c);
+    Phys<C> P = phys<C>(c);
+    const size_t n = (size_t)c->l * c->m;
+    hipLaunchKernelGGL((k_fields<S, C>), grid2(c->l, c->m, B2), B2, 0, c->s, L, P, d, d + n, d + 2 * n);
+    HIPCHK(c, hipGetLastError());
+    return BGK_OK;
+}
+
 template <class S>
 int fill_t(bgk_ctx *c, S v)
 {
@@ -649,6 +661,7 @@ int bgk_create(const bgk_params_t *p, bgk_ctx_t **out)
     c->inited = false;
     c->stage_bytes = 0;
     c->d_stage = nullptr;
+    c->d_fields = nullptr;
     if (hipSetDevice(c->dev) != hipSuccess) {
         delete c;
         return BGK_E_HIP;
@@ -715,6 +728,7 @@ int bgk_destroy(bgk_ctx_t *c)
     hipFree(c->d_red);
     hipHostFree(c->h_red);
     if (c->d_stage) hipFree(c->d_stage);
+    if (c->d_fields) hipFree(c->d_fields);
     hipEventDestroy(c->ev0);
     hipEventDestroy(c->ev1);
     hipStreamDestroy(c->s);
@@ -959,6 +973,26 @@ int bgk_draglift(bgk_ctx_t *c, int32_t i0, int32_t i1, int32_t j0, int32_t j1, d
     out[0] = c->h_red[0];
     out[1] = c->h_red[1];
     return BGK_OK;
+}
+
+int bgk_fields(bgk_ctx_t *c, float *u, float *v, float *rho)
+{
+    if (!c || !u || !v || !rho) return BGK_E_INVALID;
+    int r = check_ctx(c);
+    if (r) return r;
+    const size_t n = (size_t)c->l * c->m;
+    if (!c->d_fields) HIPCHK(c, hipMalloc((void **)&c->d_fields, 3 * n * sizeof(float)));
+    float *d = c->d_fields;
+    r = DISPATCH(c, fields_t, d);
+    if (r == BGK_OK) {
+        hipError_t e = hipMemcpyAsync(u, d, n * sizeof(float), hipMemcpyDeviceToHost, c->s);
+        if (e == hipSuccess) e = hipMemcpyAsync(v, d + n, n * sizeof(float), hipMemcpyDeviceToHost, c->s);
+        if (e == hipSuccess) e = hipMemcpyAsync(rho, d + 2 * n, n * sizeof(float), hipMemcpyDeviceToHost, c->s);
+        if (e != hipSuccess) r = fail(c, BGK_E_HIP, "bgk_fields copy: %s", hipGetErrorString(e));
+    }
+    hipError_t e = hipStreamSynchronize(c->s);
+    if (r == BGK_OK && e != hipSuccess) r = fail(c, BGK_E_HIP, "bgk_fields: %s", hipGetErrorString(e));
+    return r;
 }
 
 int bgk_storage_bytes(const bgk_ctx_t *c) { return c ? c->esize : BGK_E_INVALID; }

@@ -993,6 +993,37 @@ __global__ __launch_bounds__(DL_T) void k_draglift(Lat L, Phys<C> P, int i0, int
     }
 }
 
+// Macroscopic fields of the VTK writers (SRC/vtk_xy.binary.F90:60-140):
+// per interior node, the population sums in the storage kind (FP32 for the
+// 16-bit kinds) rounded to FP32, then u/den, w/den and den in FP32; dense
+// l*m arrays, x fastest.
+template <class S>
+struct SumKind {
+    using T = float;
+};
+template <>
+struct SumKind<double> {
+    using T = double;
+};
+template <class S, class C>
+__global__ void k_fields(Lat L, Phys<C> P, float *u, float *v, float *rho)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > L.l || j > L.m) return;
+    using T = typename SumKind<S>::T;
+    T a[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) a[k] = (T)get<S, C>(L, P, L.a, k, i, j);
+    const float us = (float)(a[0] + a[1] + a[2] - a[4] - a[5] - a[6]);
+    const float ws = (float)(a[1] + a[3] + a[5] - a[0] - a[4] - a[7]);
+    const float den = (float)((a[0] + a[1] + a[2] + a[3] + a[4] + a[5] + a[6] + a[7] + a[8]) + (T)1.0);
+    const size_t q = (size_t)(i - 1) + (size_t)(j - 1) * L.l;
+    u[q] = us / den;
+    v[q] = ws / den;
+    rho[q] = den;
+}
+
 template <class S>
 __global__ void k_pack(Lat L, const void *plane, S *dense)
 {

@@ -252,11 +252,14 @@ contains
     ctes4 = cteS
   end subroutine read_sp
 
-  ! diagnostic(itime,ivtim,icheck,itsave) (SRC/diagnostic.F90:32-103);
-  ! VTK dumps of the ivtim branch are not produced by this driver
+  ! diagnostic(itime,ivtim,icheck,itsave) (SRC/diagnostic.F90:32-103)
   subroutine diagnostic(it)
     integer, intent(in) :: it
     integer(c_int) :: rc
+    if (mod(it, ivtim) == 0) then
+       call vtk_xy_out(it)
+       call vtk_om_out(it)
+    end if
     if (mod(it, icheck) == 0) then
        call diagno_out(it)
        call probe_out(it, l/2, m/2)
@@ -269,6 +272,123 @@ contains
     end if
     if (mod(it, itsave) == 0) call save_raw(it)
   end subroutine diagnostic
+
+  ! vtk_xy_bin (SRC/vtk_xy.binary.F90:46-150): legacy VTK, text header,
+  ! big-endian FP32 blocks: velocity vectors (u, w, 0), then w, u, rho;
+  ! the fields come from bgk_fields (u/rho, w/rho, rho in FP32)
+  subroutine vtk_xy_out(it)
+    integer, intent(in) :: it
+    real(c_float), allocatable :: u(:), w(:), den(:)
+    character(len=24) :: fname
+    integer :: q
+    allocate(u(l * m), w(l * m), den(l * m))
+    call bgk_ok(ctx, bgk_fields(ctx, u, w, den), 'bgk_fields')
+    fname = 'tec_xy.xxxxxxxx.vtk'
+    write(fname(8:15), '(i8.8)') it
+    open(52, file=fname, status='replace')
+    write(52,'(A26)') '# vtk DataFile Version 2.0'
+    write(52,'(A5)') 'Campo'
+    write(52,'(A6)') 'BINARY'
+    write(52,'(A25)') 'DATASET STRUCTURED_POINTS'
+    write(52,'(A11,I10,A1,I10,A1,I10)') 'DIMENSIONS ', l, ' ', m, ' ', 1
+    write(52,'(A7,I10,A1,I10,A1,I10)') 'ORIGIN ', 1, ' ', 1, ' ', 1
+    write(52,'(A8,I10,A1,I10,A1,I10)') 'SPACING ', 1, ' ', 1, ' ', 1
+    write(52,'(A10,I10)') 'POINT_DATA ', l * m
+    write(52,'(A23)') 'VECTORS velocity float'
+    close(52)
+    call vtk_append(fname)
+    write(52) (u(q), w(q), 0.0_c_float, q = 1, l * m)
+    close(52)
+    call vtk_label(fname, 'SCALARS w float', 21)
+    call vtk_append(fname)
+    write(52) w
+    close(52)
+    call vtk_label(fname, 'SCALARS u float', 21)
+    call vtk_append(fname)
+    write(52) u
+    close(52)
+    call vtk_label(fname, 'SCALARS rho float', 21)
+    call vtk_append(fname)
+    write(52) den
+    close(52)
+    deallocate(u, w, den)
+    write(16,*) 'I/O: plane xy (vtk,binary) done'
+  end subroutine vtk_xy_out
+
+  ! vtk_om_bin (SRC/vtk_om.binary.F90:50-160): vorticity
+  ! 0.5 (u(i,j+1) - u(i,j-1)) - 0.5 (w(i+1,j) - w(i-1,j)) and the stream
+  ! function of nodes 2..l-1 x 2..m-1.  The stream function starts its
+  ! column-1 prefix at row 1 (the reference adds the halo node (1,0), a
+  ! boundary value this library does not keep between steps).
+  subroutine vtk_om_out(it)
+    integer, intent(in) :: it
+    real(c_float), allocatable :: u(:), w(:), den(:), phi(:)
+    character(len=24) :: fname
+    integer :: i, j
+    allocate(u(l * m), w(l * m), den(l * m), phi(0:m))
+    call bgk_ok(ctx, bgk_fields(ctx, u, w, den), 'bgk_fields')
+    phi(0) = 0.0
+    do j = 1, m
+       phi(j) = phi(j - 1) + u(1 + (j - 1) * l)
+    end do
+    fname = 'tec_om.xxxxxxxx.vtk'
+    write(fname(8:15), '(i8.8)') it
+    open(55, file=fname, status='replace')
+    write(55,'(A26)') '# vtk DataFile Version 2.0'
+    write(55,'(A5)') 'Campo'
+    write(55,'(A6)') 'BINARY'
+    write(55,'(A25)') 'DATASET STRUCTURED_POINTS'
+    write(55,'(A11,I10,A1,I10,A1,I10)') 'DIMENSIONS ', l - 2, ' ', m - 2, ' ', 1
+    write(55,'(A7,I10,A1,I10,A1,I10)') 'ORIGIN ', 2, ' ', 2, ' ', 1
+    write(55,'(A8,I10,A1,I10,A1,I10)') 'SPACING ', 1, ' ', 1, ' ', 1
+    write(55,'(A10,I10)') 'POINT_DATA ', (l - 2) * (m - 2)
+    write(55,'(A24)') 'SCALARS vorticity float'
+    write(55,'(A20)') 'LOOKUP_TABLE default'
+    close(55)
+    open(55, file=fname, status='old', position='append', form='unformatted', &
+         access='STREAM', convert='BIG_ENDIAN')
+    do j = 2, m - 1
+       do i = 2, l - 1
+          write(55) 0.5 * (u(i + j * l) - u(i + (j - 2) * l)) - &
+               0.5 * (w(i + 1 + (j - 1) * l) - w(i - 1 + (j - 1) * l))
+       end do
+    end do
+    close(55)
+    open(55, file=fname, status='old', position='append')
+    write(55,'(A25)') 'SCALARS stream float'
+    write(55,'(A20)') 'LOOKUP_TABLE default'
+    close(55)
+    open(55, file=fname, status='old', position='append', form='unformatted', &
+         access='STREAM', convert='BIG_ENDIAN')
+    do j = 2, m - 1
+       do i = 2, l - 1
+          phi(j) = phi(j) - w(i + (j - 1) * l)
+          write(55) phi(j)
+       end do
+    end do
+    close(55)
+    deallocate(u, w, den, phi)
+    write(16,*) 'I/O: vorticity xy (vtk) done'
+  end subroutine vtk_om_out
+
+  subroutine vtk_append(fname)
+    character(len=*), intent(in) :: fname
+    open(52, file=fname, status='old', position='append', form='unformatted', &
+         access='STREAM', convert='BIG_ENDIAN')
+  end subroutine vtk_append
+
+  ! a label line written with the reference's (Aw) edit descriptor, which
+  ! right-justifies the text in w columns
+  subroutine vtk_label(fname, label, width)
+    character(len=*), intent(in) :: fname, label
+    integer, intent(in) :: width
+    character(len=16) :: fmt
+    write(fmt, '(a,i0,a)') '(A', width, ')'
+    open(52, file=fname, status='old', position='append')
+    write(52, fmt) label
+    write(52,'(A20)') 'LOOKUP_TABLE default'
+    close(52)
+  end subroutine vtk_label
 
   ! cylinder (i-obs_x)^2 + (j-obs_y)^2 <= radius^2 as build_obs's
   ! (SRC/build.obs.F90:170-205), at a chosen centre

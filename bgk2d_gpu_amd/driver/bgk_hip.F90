@@ -57,6 +57,11 @@ module bgk_hip
        integer(c_int32_t), value :: i0, i1, j0, j1
        real(c_double), intent(out) :: out(2)
      end function
+     integer(c_int) function bgk_fields(ctx, u, v, rho) bind(c, name='bgk_fields')
+       import :: c_int, c_ptr, c_float
+       type(c_ptr), value :: ctx
+       real(c_float), intent(out) :: u(*), v(*), rho(*)
+     end function
      integer(c_int) function bgk_init(ctx, init_v, amplitude) bind(c, name='bgk_init')
        import :: c_int, c_ptr, c_int32_t, c_double
        type(c_ptr), value :: ctx

@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:396cc9bfe76a6f00
+﻿!mod$ v1 sum:75bb9703917f8f51
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ bb381bf46e508468 i __fortran_builtins
 module bgk_hip
@@ -171,6 +171,16 @@ integer(4),value::j0
 integer(4),value::j1
 real(8),intent(out)::out(1_8:2_8)
 integer(4)::bgk_draglift
+end
+end interface
+interface
+function bgk_fields(ctx,u,v,rho) bind(c,name="bgk_fields")
+import::c_ptr
+type(c_ptr),value::ctx
+real(4),intent(out)::u(1_8:*)
+real(4),intent(out)::v(1_8:*)
+real(4),intent(out)::rho(1_8:*)
+integer(4)::bgk_fields
 end
 end interface
 interface

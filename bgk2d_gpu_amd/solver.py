@@ -25,7 +25,7 @@ EXPORTS = ("bgk_create", "bgk_destroy", "bgk_last_error", "bgk_version", "bgk_co
            "bgk_build_obs", "bgk_set_obstacle", "bgk_get_obstacle", "bgk_init",
            "bgk_boundaries", "bgk_propagation", "bgk_collision", "bgk_run", "bgk_sync",
            "bgk_diagno", "bgk_varm", "bgk_prof_i", "bgk_prof_j", "bgk_probe", "bgk_draglift",
-           "bgk_check",
+           "bgk_fields", "bgk_check",
            "bgk_download", "bgk_upload", "bgk_storage_bytes", "bgk_stream", "bgk_last_run_ms",
            "bgk_batch_create", "bgk_batch_destroy", "bgk_batch_last_error", "bgk_footprint",
            "bgk_batch_add", "bgk_batch_ctx", "bgk_batch_remaining", "bgk_batch_remove",
@@ -83,6 +83,7 @@ def load(path=None):
         "bgk_prof_j": ([V, I32, P(D), P(D), P(D)], C.c_int),
         "bgk_probe": ([V, I32, I32, P(D)], C.c_int),
         "bgk_draglift": ([V, I32, I32, I32, I32, P(D)], C.c_int),
+        "bgk_fields": ([V, P(C.c_float), P(C.c_float), P(C.c_float)], C.c_int),
         "bgk_check": ([V, D], C.c_int),
         "bgk_download": ([V, I32, V], C.c_int),
         "bgk_upload": ([V, I32, V], C.c_int),
@@ -238,6 +239,13 @@ class Solver:
         out = np.zeros(2)
         self._chk(self.L.bgk_draglift(self.h, *(int(b) for b in box), _dp(out)))
         return out
+
+    def fields(self):
+        """(u, v, rho) of the VTK writers, float32 arrays of shape (ly, lx)"""
+        out = [np.zeros((self.ly, self.lx), dtype=np.float32) for _ in range(3)]
+        fp = [o.ctypes.data_as(C.POINTER(C.c_float)) for o in out]
+        self._chk(self.L.bgk_fields(self.h, *fp))
+        return tuple(out)
 
     def obstacle(self):
         out = np.zeros((self.ly, self.lx), dtype=np.int32)

@@ -164,6 +164,10 @@ int bgk_probe(bgk_ctx_t *ctx, int32_t i0, int32_t j0, double out[3]);
  * population whose neighbour is solid, in FP64.  The reference's box is
  * the cylinder centre +- (radius + 3); Cd = Fx / (U^2 radius).            */
 int bgk_draglift(bgk_ctx_t *ctx, int32_t i0, int32_t i1, int32_t j0, int32_t j1, double out[2]);
+/* Fields of the VTK writers vtk_xy_bin / vtk_om_bin (SRC/vtk_xy.binary.F90:
+ * 60-140, SRC/vtk_om.binary.F90): u/rho, v/rho and rho per interior node in
+ * FP32 (population sums in the storage kind), l*m each, x fastest.        */
+int bgk_fields(bgk_ctx_t *ctx, float *u, float *v, float *rho);
 /* 0 if every node is finite with |u| <= umax, else BGK_E_DIVERGED */
 int bgk_check(bgk_ctx_t *ctx, double umax);
 

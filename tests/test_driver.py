@@ -141,3 +141,66 @@ def test_driver_cylinder_channel_and_draglift(tmp_path, phases):
     assert dl.shape == (2, 3) and dl[-1, 0] == 200
     f = o.draglift((xc - r - 3, xc + r + 3, yc - r - 3, yc + r + 3)) / (u0 * u0 * r)
     np.testing.assert_allclose(dl[-1, 1:], f, rtol=1e-7)
+
+
+def read_vtk(path):
+    """legacy VTK as the reference writes it: text lines, then big-endian
+    FP32 blocks after each 'VECTORS'/'LOOKUP_TABLE' line"""
+    raw = open(path, "rb").read()
+    pos, header, blocks = 0, [], {}
+    npts = None
+    while pos < len(raw):
+        end = raw.index(b"\n", pos)
+        line = raw[pos:end].decode().strip()  # (Aw) right-justifies: leading blanks
+        pos = end + 1
+        header.append(line)
+        if line.startswith("POINT_DATA"):
+            npts = int(line.split()[1])
+        if line.startswith("VECTORS"):
+            n = 3 * npts
+            blocks[line.split()[1]] = np.frombuffer(raw[pos:pos + 4 * n], dtype=">f4").reshape(npts, 3)
+            pos += 4 * n
+        elif line.startswith("LOOKUP_TABLE"):
+            name = header[-2].split()[1]
+            blocks[name] = np.frombuffer(raw[pos:pos + 4 * npts], dtype=">f4")
+            pos += 4 * npts
+    return header, blocks
+
+
+@pytest.mark.gpu
+def test_driver_vtk_files(tmp_path):
+    """tec_xy / tec_om at every ivtim step, in the layout of
+    SRC/vtk_xy.binary.F90 and SRC/vtk_om.binary.F90"""
+    d = str(tmp_path)
+    l, m = 40, 30
+    with open(os.path.join(d, "bgk.input"), "w") as f:
+        f.write(INPUT.format(lx=l, ly=m, svisc=0.05, u0=0.1, itfin=100, isignal=100, itsave=100,
+                             icheck=100, irestart=0, case="LDC", prec="SINGLE", impl="FUSED",
+                             phases=0, radius=16, extra="").replace("ivtim=50000", "ivtim=50"))
+    r = run_driver(d)
+    assert r.returncode == 0, r.stdout
+    assert os.path.exists(os.path.join(d, "tec_xy.00000050.vtk"))
+    hdr, b = read_vtk(os.path.join(d, "tec_xy.00000100.vtk"))
+    assert hdr[:4] == ["# vtk DataFile Version 2.0", "Campo", "BINARY", "DATASET STRUCTURED_POINTS"]
+    raw = open(os.path.join(d, "tec_xy.00000100.vtk"), "rb").read()
+    # the reference's edit descriptors: (A23) and (A21) right-justify
+    assert b"\n VECTORS velocity float\n" in raw and b"      SCALARS w float\nLOOKUP_TABLE" in raw
+    assert hdr[4].split() == ["DIMENSIONS", str(l), str(m), "1"]
+    assert hdr[7].split() == ["POINT_DATA", str(l * m)]
+    o = po.Oracle(l, m, bc="driven", impl="fused", prec="single", svisc=0.05, u0=0.1)
+    o.step(100)
+    a = o.raw()[:, 1:-1, 1:-1].reshape(9, -1)
+    us = a[0] + a[1] + a[2] - a[4] - a[5] - a[6]
+    ws = a[1] + a[3] + a[5] - a[0] - a[4] - a[7]
+    den = (a[0] + a[1] + a[2] + a[3] + a[4] + a[5] + a[6] + a[7] + a[8]) + np.float32(1)
+    u, w = us / den, ws / den
+    assert np.array_equal(b["velocity"][:, 0], u) and np.array_equal(b["velocity"][:, 1], w)
+    assert not b["velocity"][:, 2].any()
+    assert np.array_equal(b["w"], w) and np.array_equal(b["u"], u) and np.array_equal(b["rho"], den)
+    hdr, b = read_vtk(os.path.join(d, "tec_om.00000100.vtk"))
+    assert hdr[4].split() == ["DIMENSIONS", str(l - 2), str(m - 2), "1"]
+    U, Wf = u.reshape(m, l), w.reshape(m, l)
+    vort = (np.float32(0.5) * (U[2:, 1:-1] - U[:-2, 1:-1])
+            - np.float32(0.5) * (Wf[1:-1, 2:] - Wf[1:-1, :-2]))
+    assert np.array_equal(b["vorticity"], vort.ravel())
+    assert b["stream"].shape == ((l - 2) * (m - 2),) and np.isfinite(b["stream"]).all()
