@@ -132,6 +132,7 @@ Phys<C> phys(const bgk_ctx *c)
     P.channel_force = c->p.bc == BGK_BC_CHANNEL ? 1 : 0;
     P.form = c->p.form;
     P.les = c->p.les;
+    P.trick1 = c->p.trick1;
     P.ctes = (C)c->p.cte_s;
     P.rounding = (is_fp16(c->p.prec) || is_bf16(c->p.prec)) ? c->p.rounding : 0;
     return P;
@@ -205,8 +206,13 @@ int launch_boundaries(bgk_ctx *c)
         pass(BP_STEP_FACE, 1, L.stepy, 0);
         break;
     default:
-        pass(BP_DRV_X, 1, m, 0);
-        pass(BP_WALL_Y, 1, l, 1);
+        if (c->p.trick1) {  // SRC/bcond.driven.F90:43-81: side walls after the lid
+            pass(BP_WALL_Y, 1, l, 1);
+            pass(BP_DRV_X, 1, m, 0);
+        } else {
+            pass(BP_DRV_X, 1, m, 0);
+            pass(BP_WALL_Y, 1, l, 1);
+        }
         break;
     }
     if (c->p.obstacle && c->nobs > 0) {
@@ -631,7 +637,10 @@ int bgk_create(const bgk_params_t *p, bgk_ctx_t **out)
     if (p->form != 0 && p->form != 1) return BGK_E_INVALID;
     if (p->rounding != 0 && p->rounding != 1) return BGK_E_INVALID;
     if (p->tblock < 0 || p->tblock > 2) return BGK_E_INVALID;
-    if ((p->les != 0 && p->les != 1) || p->reserved != 0 || !(p->cte_s >= 0.0)) return BGK_E_INVALID;
+    if ((p->les != 0 && p->les != 1) || !(p->cte_s >= 0.0)) return BGK_E_INVALID;
+    // TRICK1 is the driven cavity's and, as in the reference, square only
+    if (p->trick1 != 0 && (p->trick1 != 1 || p->bc != BGK_BC_DRIVEN || p->lx != p->ly)) return BGK_E_INVALID;
+    if (p->obstacle < 0 || p->obstacle > 3) return BGK_E_INVALID;
     bgk_ctx *c = new bgk_ctx();
     c->p = *p;
     c->l = p->lx;
@@ -769,7 +778,8 @@ static int set_mask(bgk_ctx *c, const int32_t *obs)
     c->jmin = jmin;
     c->jmax = jmax;
     c->nobs = n;
-    c->p.obstacle = n > 0 ? 1 : 0;  // a mask turns bcond_obs on, as the OBSTACLE build
+    if (n == 0) c->p.obstacle = 0;             // a mask turns bcond_obs on, as the
+    else if (c->p.obstacle == 0) c->p.obstacle = 1;  // OBSTACLE build; keeps a shape code
     c->tb_h = 0;                    // the two-step kernel is sized per instance
     c->h_obs.assign(obs, obs + (size_t)c->l * c->m);
     HIPCHK(c, hipMemcpyAsync(c->d_obs, h.data(), h.size(), hipMemcpyHostToDevice, c->s));
@@ -783,22 +793,29 @@ int bgk_build_obs(bgk_ctx_t *c)
 {
     int r = check_ctx(c);
     if (r) return r;
-    // SRC/build.obs.F90:170-205: obs = 1 where (icoord-i)^2+(jcoord-j)^2 <= R*R
-    // with the squared distance converted to mykind
-    const int l = c->l, m = c->m, ic = 2 * l / 5, jc = m / 2;
+    // SRC/build.obs.F90:49-205, centred at (2l/5, m/2), comparisons in
+    // mykind: obstacle 1 (and 0) the cylinder (icoord-i)^2+(jcoord-j)^2 <=
+    // R*R (default branch), 2 the square of half side radius (SQUARE), 3 the
+    // cylinder's rows m/2..m (HALF_CIRCLE)
+    const int l = c->l, m = c->m, ic = 2 * l / 5, jc = m / 2, shape = c->p.obstacle;
     std::vector<int32_t> obs((size_t)l * m, 0);
     const bool dbl = c->p.prec == BGK_PREC_DOUBLE || c->p.prec == BGK_PREC_MIXED2;
     for (int j = 1; j <= m; j++)
         for (int i = 1; i <= l; i++) {
             long long d2i = (long long)(ic - i) * (ic - i) + (long long)(jc - j) * (jc - j);
             bool in;
-            if (dbl) {
+            if (shape == 2) {
+                // integer bounds against a real half size: exact for these sizes
+                const double h = (double)c->p.radius;
+                in = j >= jc - h && j <= jc + h && i >= ic - h && i <= ic + h;
+            } else if (dbl) {
                 double R = (double)c->p.radius * 1.0;
                 in = (double)(int)d2i <= R * R;
             } else {
                 float R = (float)c->p.radius * 1.0f;
                 in = (float)(int)d2i <= R * R;
             }
+            if (shape == 3 && j < m / 2) in = false;
             obs[(size_t)(i - 1) + (size_t)(j - 1) * l] = in ? 1 : 0;
         }
     return set_mask(c, obs.data());

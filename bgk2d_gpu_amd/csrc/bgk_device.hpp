@@ -153,6 +153,7 @@ struct Phys {
     int form;               // 0: reference operation order; 1: deviation form
     int rounding;           // FP16/BF16 stores: 0 nearest-even, 1 stochastic
     int les;                // 1: Smagorinsky LES relaxation rate per node
+    int trick1;             // 1: DRIVEN top corners take the side walls' copy
     C ctes;                 // LES: Smagorinsky constant cteS
 };
 

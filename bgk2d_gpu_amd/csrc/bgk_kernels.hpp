@@ -489,6 +489,9 @@ __device__ C folded_a(const A &a, const Phys<C> &P, int l, int m, int k, int i, 
     }
     // wall bounded in y: DRIVEN, CHANNEL, COUETTE
     const int o = opp_of(k);
+    // TRICK1 corner order (SRC/bcond.driven.F90:43-81): the side walls are
+    // written after the lid, so the two top corners hold plain copies
+    if (BC == BK_DRIVEN && sj > m && P.trick1 && (si < 1 || si > l)) return a(o, i, j);
     if (sj > m && (BC == BK_DRIVEN || BC == BK_COUETTE)) {
         C own = a(o, i, j);
         const int c = cx_of(k);

@@ -29,6 +29,9 @@
 !                    DRAG build does (SRC/diagnostic.F90:80-82)
 !     les            1: Smagorinsky LES collision, as the LES build does,
 !                    with &parameters cteS
+!     trick1         1: driven-cavity corner order of the TRICK1 build
+!     shape          obstacle of VKS/CYL: 'CYLINDER' (default), 'SQUARE',
+!                    'HALF_CIRCLE' (the SQUARE / HALF_CIRCLE builds)
 program bgk2d_hip
   use iso_c_binding
   use bgk_hip
@@ -39,9 +42,10 @@ program bgk2d_hip
   real(8) :: svisc, u0, cteS
   character(len=16) :: case_name, precision, implementation
   real(8) :: scale, amplitude
-  integer :: phases, device, form, rounding, obs_x, obs_y, drag, les
+  integer :: phases, device, form, rounding, obs_x, obs_y, drag, les, trick1
+  character(len=16) :: shape
   namelist /gpu/ case_name, precision, implementation, scale, amplitude, phases, device, &
-       form, rounding, obs_x, obs_y, drag, les
+       form, rounding, obs_x, obs_y, drag, les, trick1, shape
 
   type(bgk_params_t) :: p
   type(c_ptr) :: ctx
@@ -73,6 +77,8 @@ program bgk2d_hip
   obs_y = -1
   drag = 0
   les = 0
+  trick1 = 0
+  shape = 'CYLINDER' 
   open(15, file='bgk.input', status='old', iostat=ios)
   if (ios /= 0) error stop 'bgk.input not found'
   read(15, gpu, iostat=ios)
@@ -93,7 +99,7 @@ program bgk2d_hip
   p%rounding = rounding
   p%tblock = 0
   p%les = les
-  p%reserved = 0
+  p%trick1 = trick1
   p%obstacle = 0
   p%scale = scale
   select case (trim(case_name))
@@ -106,6 +112,14 @@ program bgk2d_hip
   case ('CYL'); p%bc = BGK_BC_OPEN; p%obstacle = 1
   case default; error stop 'unknown case_name'
   end select
+  if (p%obstacle == 1) then
+     select case (trim(shape))
+     case ('CYLINDER')
+     case ('SQUARE'); p%obstacle = 2
+     case ('HALF_CIRCLE'); p%obstacle = 3
+     case default; error stop 'unknown shape'
+     end select
+  end if
   select case (trim(precision))
   case ('SINGLE'); p%prec = BGK_PREC_SINGLE
   case ('DOUBLE'); p%prec = BGK_PREC_DOUBLE

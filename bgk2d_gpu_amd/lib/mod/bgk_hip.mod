@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:75bb9703917f8f51
+﻿!mod$ v1 sum:6cd3fa8ba3683b1f
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ bb381bf46e508468 i __fortran_builtins
 module bgk_hip
@@ -109,7 +109,7 @@ integer(4)::form
 integer(4)::rounding
 integer(4)::tblock
 integer(4)::les
-integer(4)::reserved
+integer(4)::trick1
 real(8)::svisc
 real(8)::u0
 real(8)::scale

@@ -81,7 +81,10 @@ typedef struct bgk_params {
     int32_t lx, ly;       /* interior size (namelist lx, ly)                */
     int32_t ipad, jpad;   /* namelist padding; accepted, layout is pitched  */
     int32_t bc;           /* enum bgk_bc                                    */
-    int32_t obstacle;     /* 1: obstacle mask + bcond_obs (VKS build)       */
+    int32_t obstacle;     /* bcond_obs with the build_obs mask: 1 cylinder
+                             (VKS build), 2 square (SQUARE), 3 half
+                             cylinder (HALF_CIRCLE); a bgk_set_obstacle
+                             mask turns a 0 into 1                          */
     int32_t impl;         /* enum bgk_impl                                  */
     int32_t prec;         /* enum bgk_prec                                  */
     int32_t radius;       /* cylinder radius (namelist radius)              */
@@ -100,7 +103,8 @@ typedef struct bgk_params {
                              steps per kernel; results are identical       */
     int32_t les;          /* 1: Smagorinsky LES collision, the -DLES build
                              (SRC/col_MC.F90:178-230)                       */
-    int32_t reserved;     /* must be 0                                      */
+    int32_t trick1;       /* 1: driven-cavity corner order of the TRICK1 build
+                             (SRC/bcond.driven.F90:43-81; lx == ly)         */
     double svisc;         /* namelist svisc (kinematic viscosity)           */
     double u0;            /* namelist u0 (lid speed, or channel force scale)*/
     double scale;         /* FP16/BF16 storage scale (power of two, >= 1)   */
@@ -122,8 +126,9 @@ int bgk_version(void);
 /* omega, fgrad, u00, force (= u00/6) in the compute kind, widened */
 int bgk_constants(const bgk_ctx_t *ctx, double out[4]);
 
-/* Cylinder of SRC/build.obs.F90:170-205 (centre 2l/5, m/2).  The mask
- * must stay at least two nodes from every wall (else BGK_E_INVALID).    */
+/* Mask of SRC/build.obs.F90:49-205 for params.obstacle (cylinder, square,
+ * half cylinder; centre 2l/5, m/2).  The mask must stay at least two
+ * nodes from every wall (else BGK_E_INVALID).                           */
 int bgk_build_obs(bgk_ctx_t *ctx);
 /* Caller-supplied mask obs(1:l,1:m) as l*m int32, x fastest (1 = solid). */
 int bgk_set_obstacle(bgk_ctx_t *ctx, const int32_t *obs);

@@ -37,7 +37,9 @@ enum { ORC_ORIGINAL = 0, ORC_FUSED = 1 };
 typedef struct {
     int32_t lx, ly;        /* interior size l, m                          */
     int32_t bc;            /* ORC_BC_*                                    */
-    int32_t obstacle;      /* 1: cylinder mask + bcond_obs (VKS)          */
+    int32_t obstacle;      /* build_obs shape + bcond_obs: 1 cylinder (VKS),
+                              2 square (SQUARE), 3 half cylinder
+                              (HALF_CIRCLE)                               */
     int32_t impl;          /* ORC_ORIGINAL / ORC_FUSED                    */
     int32_t prec;          /* ORC_SINGLE ...                              */
     int32_t radius;        /* cylinder radius (namelist `radius`)         */
@@ -47,7 +49,7 @@ typedef struct {
     int32_t form;          /* 0: reference operation order; 1: deviation  */
     int32_t rounding;      /* half/bf16 stores: 0 nearest, 1 stochastic   */
     int32_t les;           /* 1: Smagorinsky LES collision (-DLES build)  */
-    int32_t reserved;
+    int32_t trick1;        /* 1: driven corner order of the TRICK1 build  */
     double svisc, u0;      /* namelist values                             */
     double amplitude;      /* TGV amplitude (reference hard-codes 0.1)    */
     double scale;          /* storage scale for half/bf16 (power of two)  */

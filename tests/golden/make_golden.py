@@ -55,6 +55,15 @@ CASES = [
     ("DOUBLE_FUSED_LDCLES", 24, 20, 0.005, 0.1, 60, 16),
     ("MIXED2_FUSED_LDCLES", 24, 20, 0.005, 0.1, 60, 16),
     ("SINGLE_FUSED_VKSLES", 40, 24, 0.005, 0.1, 60, 4),
+    # TRICK1 corner order of bcond_driven (square cavity), SQUARE and
+    # HALF_CIRCLE obstacles of build_obs
+    ("SINGLE_FUSED_LDCT1", 24, 24, 0.1, 0.1, 60, 16),
+    ("SINGLE_ORIG_LDCT1", 24, 24, 0.1, 0.1, 60, 16),
+    ("DOUBLE_FUSED_LDCT1", 24, 24, 0.1, 0.1, 60, 16),
+    ("SINGLE_FUSED_VKSSQ", 40, 24, 0.05, 0.1, 60, 4),
+    ("SINGLE_ORIG_VKSSQ", 40, 24, 0.05, 0.1, 60, 4),
+    ("SINGLE_FUSED_VKSHC", 40, 24, 0.05, 0.1, 60, 5),
+    ("DOUBLE_FUSED_VKSHC", 40, 24, 0.05, 0.1, 60, 5),
 ]
 
 

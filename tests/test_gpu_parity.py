@@ -47,7 +47,8 @@ def fixture_solver(name, impl_override=None):
     impl = impl_override or ("fused" if impl == "FUSED" else "original")
     s = Solver(lx, ly, bc=bc, impl=impl, prec=GPU_PREC[p], svisc=sv, u0=u0, obstacle=obst,
                radius=rad, mixed1_quirk=1 if p == "MIXED1" else 0,
-               les=int(case in po.LES_CASES), cte_s=po.CTE_S_REF)
+               les=int(case in po.LES_CASES), cte_s=po.CTE_S_REF,
+               trick1=int(case in po.TRICK1_CASES))
     return s, ns
 
 
