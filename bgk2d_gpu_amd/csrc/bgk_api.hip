@@ -428,8 +428,9 @@ int graph_for(bgk_ctx *c, hipGraphExec_t *out)
     return BGK_OK;
 }
 
+// enqueue nsteps steps on the context's stream, without waiting
 template <class S, class C>
-int run_t(bgk_ctx *c, int nsteps)
+int enqueue_t(bgk_ctx *c, int nsteps)
 {
     if (foldable(c) && c->tb_h == 0) {
         // size the two-step kernel (which decides whether it is used)
@@ -437,7 +438,6 @@ int run_t(bgk_ctx *c, int nsteps)
         int r = c->p.obstacle && c->nobs > 0 ? size_tb_o<S, C, true>(c) : size_tb_o<S, C, false>(c);
         if (r != BGK_OK) return r;
     }
-    HIPCHK(c, hipEventRecord(c->ev0, c->s));
     int t = 0;
     if (nsteps >= GRAPH_STEPS) {
         hipGraphExec_t g;
@@ -445,7 +445,14 @@ int run_t(bgk_ctx *c, int nsteps)
         if (r != BGK_OK) return r;
         for (; t + GRAPH_STEPS <= nsteps; t += GRAPH_STEPS) HIPCHK(c, hipGraphLaunch(g, c->s));
     }
-    int r = launch_steps<S, C>(c, nsteps - t);
+    return launch_steps<S, C>(c, nsteps - t);
+}
+
+template <class S, class C>
+int run_t(bgk_ctx *c, int nsteps)
+{
+    HIPCHK(c, hipEventRecord(c->ev0, c->s));
+    int r = enqueue_t<S, C>(c, nsteps);
     if (r != BGK_OK) return r;
     HIPCHK(c, hipEventRecord(c->ev1, c->s));
     HIPCHK(c, hipEventSynchronize(c->ev1));
@@ -1079,10 +1086,13 @@ struct bgk_batch {
     };
     std::vector<Slot> slots;
     bgk::Member *d_members, *h_members;
-    int2 *d_tiles, *h_tiles;
-    int *d_ntiles, *h_ntiles;
+    int2 *d_tiles, *h_tiles;  // task table: (member, first tile)
+    int *d_ntiles, *h_ntiles;  // per group: task counts, then first tasks
     size_t tiles_cap;
-    std::map<int, hipGraphExec_t> graphs;
+    std::map<long long, hipGraphExec_t> graphs;  // key: (chunk, group, full grid)
+    std::map<long long, int> uses;
+    hipStream_t gs[bgk::NGROUPS];          // one stream per group, made on first use
+    hipEvent_t gev[bgk::NGROUPS];
     int grid;
     hipEvent_t ev0, ev1;
     double last_ms;
@@ -1109,6 +1119,73 @@ int bfail(bgk_batch_t *b, int code, const char *fmt, ...)
     } while (0)
 
 // tile codes: 1 = 64x4, 2 = 128x2, 3 = 256x1, 4 = 32x8
+// tiles of a member: (1 << tw_log2) columns -- node pairs for the 16-bit
+// kinds -- by th rows
+int member_tiles_x(const bgk_ctx *c, int tw_log2)
+{
+    const int w = (1 << tw_log2) * (c->esize == 2 ? 2 : 1);
+    return (c->l + w - 1) / w;
+}
+size_t member_tiles(const bgk_ctx *c, int tw_log2, int th)
+{
+    return (size_t)member_tiles_x(c, tw_log2) * (size_t)((c->m + th - 1) / th);
+}
+// A member of at least SOLO_NODES nodes (2048^2) keeps the chip busy by
+// itself for a whole step: it advances with bgk_run's own kernels (the
+// two-step kernel where that is faster: 122 k against 68 k MLUPS at 4096^2
+// FP32) instead of through the task table.  Smaller members run faster
+// together in the batch kernels.  Both paths give the same bits.
+constexpr size_t SOLO_NODES = (size_t)1 << 22;
+// tasks up to which a group's chain launches SMALL_GROUP blocks
+constexpr size_t SMALL_GROUP = 512;
+bool member_solo(const bgk_ctx *c) { return (size_t)c->l * (size_t)c->m >= SOLO_NODES; }
+
+// entries of the batch task table for a member
+size_t member_tasks(const bgk_ctx *c, int tw_log2, int th)
+{
+    if (member_solo(c)) return 0;
+    return (member_tiles(c, tw_log2, th) + bgk::TASK_TILES - 1) / bgk::TASK_TILES;
+}
+int member_kind(const bgk_ctx *c)
+{
+    return c->p.prec == BGK_PREC_FP16 ? bgk::MK_F16 : (c->p.prec == BGK_PREC_BF16 ? bgk::MK_BF16 : bgk::MK_F32);
+}
+int member_group(const bgk_ctx *c)
+{
+    int v = 0;
+    while (v < 5 && bgk::BATCH_BCS[v] != c->p.bc) v++;
+    return member_kind(c) * 6 + v;
+}
+
+// one step of a group's members
+extern "C++" {
+template <class S>
+void launch_batch_s(bgk_batch_t *b, int g, int nb, int t)
+{
+    const dim3 grid(nb), blk(256);
+    const bgk::Member *dm = b->d_members;
+    const int2 *dt = b->d_tiles;
+    const int *meta = b->d_ntiles;
+    hipStream_t s = b->gs[g];
+    switch (bgk::BATCH_BCS[g % 6]) {
+    case bgk::BK_PERIODIC: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_PERIODIC>), grid, blk, 0, s, dm, dt, meta, g, t); break;
+    case bgk::BK_CHANNEL: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_CHANNEL>), grid, blk, 0, s, dm, dt, meta, g, t); break;
+    case bgk::BK_INFLOW: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_INFLOW>), grid, blk, 0, s, dm, dt, meta, g, t); break;
+    case bgk::BK_COUETTE: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_COUETTE>), grid, blk, 0, s, dm, dt, meta, g, t); break;
+    case bgk::BK_OPEN: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_OPEN>), grid, blk, 0, s, dm, dt, meta, g, t); break;
+    default: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_DRIVEN>), grid, blk, 0, s, dm, dt, meta, g, t); break;
+    }
+}
+void launch_batch(bgk_batch_t *b, int g, int nb, int t)
+{
+    switch (g / 6) {
+    case bgk::MK_F16: launch_batch_s<_Float16>(b, g, nb, t); break;
+    case bgk::MK_BF16: launch_batch_s<bgk::bf16_t>(b, g, nb, t); break;
+    default: launch_batch_s<float>(b, g, nb, t); break;
+    }
+}
+}  // extern "C++"
+
 bool tile_shape(int code, int *tw_log2, int *th)
 {
     switch (code) {
@@ -1164,13 +1241,18 @@ int bgk_batch_create(int32_t device, double budget_bytes, int32_t max_members, b
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
         cus = 256;
     b->grid = cus * 8;
+    for (int g = 0; g < bgk::NGROUPS; g++) {
+        b->gs[g] = nullptr;
+        b->gev[g] = nullptr;
+    }
     if (hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess) {
         delete b;
         return BGK_E_HIP;
     }
     if (hipMalloc(&b->d_members, sizeof(bgk::Member) * max_members) != hipSuccess ||
         hipHostMalloc(&b->h_members, sizeof(bgk::Member) * max_members) != hipSuccess ||
-        hipMalloc(&b->d_ntiles, sizeof(int)) != hipSuccess || hipHostMalloc(&b->h_ntiles, sizeof(int)) != hipSuccess) {
+        hipMalloc(&b->d_ntiles, sizeof(int) * 2 * bgk::NGROUPS) != hipSuccess ||
+        hipHostMalloc(&b->h_ntiles, sizeof(int) * 2 * bgk::NGROUPS) != hipSuccess) {
         (void)hipStreamDestroy(b->s);
         delete b;
         return BGK_E_NOMEM;
@@ -1197,6 +1279,11 @@ int bgk_batch_destroy(bgk_batch_t *b)
     (void)hipHostFree(b->h_ntiles);
     (void)hipEventDestroy(b->ev0);
     (void)hipEventDestroy(b->ev1);
+    for (int g = 0; g < bgk::NGROUPS; g++) {
+        if (b->gs[g]) (void)hipStreamSynchronize(b->gs[g]);
+        if (b->gev[g]) (void)hipEventDestroy(b->gev[g]);
+        if (b->gs[g]) (void)hipStreamDestroy(b->gs[g]);
+    }
     (void)hipStreamDestroy(b->s);
     delete b;
     return BGK_OK;
@@ -1290,8 +1377,7 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
         const auto &sl = b->slots[q];
         if (!sl.ctx || sl.remaining <= 0) continue;
         const bgk_ctx *c = sl.ctx;
-        const int tw = 1 << sl.tw_log2;
-        nt += (size_t)((c->l + tw - 1) / tw) * (size_t)((c->m + sl.th - 1) / sl.th);
+        nt += member_tasks(c, sl.tw_log2, sl.th);
         members_slot.push_back((int)q);
         nm++;
     }
@@ -1302,56 +1388,106 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
     }
     int r = ensure_tiles(b, nt);
     if (r) return r;
-    size_t t = 0;
+    // the task table holds the groups one after another
+    constexpr int G = bgk::NGROUPS;
+    size_t cnt[G] = {}, pos[G];
+    for (int mi = 0; mi < nm; mi++) {
+        const auto &sl = b->slots[members_slot[mi]];
+        cnt[member_group(sl.ctx)] += member_tasks(sl.ctx, sl.tw_log2, sl.th);
+    }
+    for (int g = 0, acc = 0; g < G; g++) {
+        pos[g] = (size_t)acc;
+        b->h_ntiles[g] = (int)cnt[g];
+        b->h_ntiles[G + g] = acc;
+        acc += (int)cnt[g];
+    }
     for (int mi = 0; mi < nm; mi++) {
         const auto &sl = b->slots[members_slot[mi]];
         const bgk_ctx *c = sl.ctx;
         bgk::Member &d = b->h_members[mi];
+        // offsets from the table's device address (see bgk::Member)
+        const auto rel = [b](void *q) { return (long long)((uintptr_t)q - (uintptr_t)b->d_members); };
         for (int k = 0; k < 8; k++) {
-            d.set[0][k] = c->plane[k];
-            d.set[1][k] = c->plane[8 + k];
+            d.set[0][k] = rel(c->plane[k]);
+            d.set[1][k] = rel(c->plane[8 + k]);
         }
-        d.r[0] = c->plane[16];
-        d.r[1] = c->plane[17];
+        d.r[0] = rel(c->plane[16]);
+        d.r[1] = rel(c->plane[17]);
         d.rparity0 = c->rcur;
         d.l = c->l;
         d.m = c->m;
         d.off = c->off;
         d.rs = c->pitch;
-        d.store = c->p.prec == BGK_PREC_FP16 ? bgk::MK_F16 : (c->p.prec == BGK_PREC_BF16 ? bgk::MK_BF16 : bgk::MK_F32);
+        d.store = member_kind(c);
         d.bc = c->p.bc;  // BGK_BC_* and BcKind share the numbering of the foldable variants
-        const int tw = 1 << sl.tw_log2;
-        d.tiles_x = (c->l + tw - 1) / tw;
+        d.tiles_x = member_tiles_x(c, sl.tw_log2);
         d.tw_log2 = sl.tw_log2;
         d.th = sl.th;
         d.parity0 = c->cur;
         d.remaining = (int)(sl.remaining < chunk ? sl.remaining : chunk);
         d.P = phys<float>(c);
-        const int ntm = d.tiles_x * ((c->m + sl.th - 1) / sl.th);
-        for (int q = 0; q < ntm; q++) b->h_tiles[t++] = make_int2(mi, q);
+        d.ntiles = (int)member_tiles(c, sl.tw_log2, sl.th);
+        if (member_solo(c)) continue;
+        size_t &t = pos[member_group(c)];
+        for (int q = 0; q < d.ntiles; q += bgk::TASK_TILES) b->h_tiles[t++] = make_int2(mi, q);
     }
-    *b->h_ntiles = (int)nt;
     BHIPCHK(b, hipMemcpyAsync(b->d_members, b->h_members, sizeof(bgk::Member) * nm, hipMemcpyHostToDevice, b->s));
     BHIPCHK(b, hipMemcpyAsync(b->d_tiles, b->h_tiles, sizeof(int2) * nt, hipMemcpyHostToDevice, b->s));
-    BHIPCHK(b, hipMemcpyAsync(b->d_ntiles, b->h_ntiles, sizeof(int), hipMemcpyHostToDevice, b->s));
-    hipGraphExec_t g;
-    auto it = b->graphs.find(chunk);
-    if (it == b->graphs.end()) {
-        BHIPCHK(b, hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
-        for (int tt = 0; tt < chunk; tt++)
-            hipLaunchKernelGGL(bgk::k_batch, dim3(b->grid), dim3(256), 0, b->s, (const bgk::Member *)b->d_members,
-                               (const int2 *)b->d_tiles, (const int *)b->d_ntiles, tt);
-        hipGraph_t gr = nullptr;
-        BHIPCHK(b, hipStreamEndCapture(b->s, &gr));
-        hipError_t e = hipGraphInstantiate(&g, gr, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(gr);
-        if (e != hipSuccess) return bfail(b, BGK_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-        b->graphs[chunk] = g;
-    } else {
-        g = it->second;
-    }
+    BHIPCHK(b, hipMemcpyAsync(b->d_ntiles, b->h_ntiles, sizeof(int) * 2 * G, hipMemcpyHostToDevice, b->s));
     BHIPCHK(b, hipEventRecord(b->ev0, b->s));
-    BHIPCHK(b, hipGraphLaunch(g, b->s));
+    // The large members run one after another: their kernels are sized to
+    // fill the chip in one round, and sharing it would leave a second,
+    // half-empty round.
+    hipEvent_t prev = b->ev0;
+    for (int mi = 0; mi < nm; mi++) {
+        bgk_ctx *c = b->slots[members_slot[mi]].ctx;
+        if (!member_solo(c)) continue;
+        BHIPCHK(b, hipStreamWaitEvent(c->s, prev, 0));
+        const int rs = DISPATCH(c, enqueue_t, b->h_members[mi].remaining);  // flips c->cur, c->rcur
+        if (rs != BGK_OK) return bfail(b, rs, "member %d: %s", members_slot[mi], bgk_last_error(c));
+        BHIPCHK(b, hipEventRecord(c->ev1, c->s));
+        prev = c->ev1;
+    }
+    BHIPCHK(b, hipStreamWaitEvent(b->s, prev, 0));
+    // each group's chain of `chunk` steps replays from its own graph on its
+    // own stream, after the tables are in place; the groups overlap
+    for (int g = 0; g < G; g++) {
+        if (!cnt[g]) continue;
+        if (!b->gs[g]) {
+            BHIPCHK(b, hipStreamCreateWithFlags(&b->gs[g], hipStreamNonBlocking));
+            BHIPCHK(b, hipEventCreateWithFlags(&b->gev[g], hipEventDisableTiming));
+        }
+        // a group with few tasks launches fewer blocks, leaving the rest of
+        // the chip to the other groups' chains
+        const int nb = cnt[g] <= SMALL_GROUP ? SMALL_GROUP : b->grid;
+        const long long key = ((long long)chunk * G + g) * 2 + (nb == b->grid);
+        BHIPCHK(b, hipStreamWaitEvent(b->gs[g], prev, 0));
+        // a chain seen for the first time is launched directly; its graph
+        // is captured when it comes again (capture and instantiation cost
+        // more than the launches of one chunk)
+        if (++b->uses[key] < 2) {
+            for (int tt = 0; tt < chunk; tt++) launch_batch(b, g, nb, tt);
+            BHIPCHK(b, hipGetLastError());
+        } else {
+            hipGraphExec_t ge;
+            auto it = b->graphs.find(key);
+            if (it == b->graphs.end()) {
+                BHIPCHK(b, hipStreamBeginCapture(b->gs[g], hipStreamCaptureModeThreadLocal));
+                for (int tt = 0; tt < chunk; tt++) launch_batch(b, g, nb, tt);
+                hipGraph_t gr = nullptr;
+                BHIPCHK(b, hipStreamEndCapture(b->gs[g], &gr));
+                hipError_t e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(gr);
+                if (e != hipSuccess) return bfail(b, BGK_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+                b->graphs[key] = ge;
+            } else {
+                ge = it->second;
+            }
+            BHIPCHK(b, hipGraphLaunch(ge, b->gs[g]));
+        }
+        BHIPCHK(b, hipEventRecord(b->gev[g], b->gs[g]));
+        BHIPCHK(b, hipStreamWaitEvent(b->s, b->gev[g], 0));
+    }
     BHIPCHK(b, hipEventRecord(b->ev1, b->s));
     BHIPCHK(b, hipEventSynchronize(b->ev1));
     float ms = 0.f;
@@ -1361,8 +1497,10 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
     for (int mi = 0; mi < nm; mi++) {
         auto &sl = b->slots[members_slot[mi]];
         const int steps = b->h_members[mi].remaining;
-        sl.ctx->cur = (sl.ctx->cur + steps) & 1;
-        sl.ctx->rcur = (sl.ctx->rcur + steps) & 1;
+        if (!member_solo(sl.ctx)) {
+            sl.ctx->cur = (sl.ctx->cur + steps) & 1;
+            sl.ctx->rcur = (sl.ctx->rcur + steps) & 1;
+        }
         sl.remaining -= steps;
         sl.done += steps;
         if (sl.remaining == 0) nfin++;

@@ -600,12 +600,9 @@ __global__ __launch_bounds__(256) void k_fused(Lat L, Phys<C> P)
 // first interior node is 128-byte aligned); the x-shifted pulls start at
 // an odd element and are assembled from the two aligned dwords around them.
 template <class S, class C, int BC, bool OBS>
-__global__ __launch_bounds__(256) void k_fused2(Lat L, Phys<C> P)
+__device__ __forceinline__ void fused_pair(const Lat &L, const Phys<C> &P, int i0, int j)
 {
     static_assert(sizeof(S) == 2, "pairs of 16-bit elements");
-    const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * 2 + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    if (i0 > L.l || j > L.m) return;
     // With the column-l / inlet equilibria of INFLOW and OPEN, the boundary
     // code inlined twice below would raise the register count of every pair
     // (to ~150): there, pairs that touch an edge or the obstacle take the
@@ -656,6 +653,15 @@ __global__ __launch_bounds__(256) void k_fused2(Lat L, Phys<C> P)
         else
             __builtin_nontemporal_store((uint16_t)(ow[k] & 0xffffu), (uint16_t *)dst);
     }
+}
+
+template <class S, class C, int BC, bool OBS>
+__global__ __launch_bounds__(256) void k_fused2(Lat L, Phys<C> P)
+{
+    const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * 2 + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i0 > L.l || j > L.m) return;
+    fused_pair<S, C, BC, OBS>(L, P, i0, j);
 }
 
 // ------------------------------------------------ two steps per HBM pass

@@ -2,83 +2,89 @@
 //
 // Many independent solves ("members", each a bgk_ctx with its own size,
 // boundary variant and storage kind) stay resident in HBM and advance
-// together: one launch of k_batch performs one FUSED step of every active
-// member.  Blocks grid-stride over a device-resident tile table; a tile
-// names a member and a 256-node tile of it, and the block runs exactly the
-// per-node code of k_fused (fused_node), so a member's populations are
-// bit-identical whether it runs alone or in any mix.  Each launch takes the
-// index t of the step inside a chunk; a member whose remaining step count is
-// <= t sits the launch out, and its FUSED parity is (parity0 + t) & 1 (the
-// same for the two pop-19 planes from rparity0).  The
-// host rewrites the small member/tile tables between chunks (admission and
-// retirement), never during one, so chunks replay from HIP graphs.
+// together.  Members fall into groups by (storage kind, boundary variant);
+// one launch of k_batch<S, BC> performs one FUSED step of every active
+// member of its group, so each kernel carries only its own group's code and
+// registers (one kernel for every group needed up to 142 VGPRs and ran the
+// large members 15-25 % slower than the one-step kernels).  Blocks
+// grid-stride over the group's part of a device-resident task table; a task
+// names a member and a run of up to TASK_TILES consecutive tiles of it (a
+// tile is 256 nodes, or 256 node pairs for the 16-bit kinds), so the
+// member's description is fetched once per task and the tile loop issues no
+// dependent scalar loads.  The block runs exactly the per-node code of the
+// one-step kernels (fused_node; fused_pair for 16-bit storage), so a
+// member's populations are bit-identical whether it runs alone or in any
+// mix.  Each launch takes the index t of the step inside a chunk; a member
+// whose remaining step count is <= t sits the launch out, and its FUSED
+// parity is (parity0 + t) & 1 (the same for the two pop-19 planes from
+// rparity0).  The host rewrites the small member/task tables between chunks
+// (admission and retirement), never during one, so chunks replay from HIP
+// graphs, one chain per group on its own stream.
 #pragma once
 #include "bgk_kernels.hpp"
 
 namespace bgk {
 
-// (storage kind, boundary) pairs the batch kernel serves
+// storage kinds the batch kernels serve
 enum MemberKind {
     MK_F32 = 0,   // fp32 storage + compute
     MK_F16 = 1,   // IEEE half storage, fp32 compute
     MK_BF16 = 2,  // bfloat16 storage, fp32 compute
 };
 
+// boundary variants the batch kernels serve, in group order
+constexpr int BATCH_BCS[6] = {BK_DRIVEN, BK_PERIODIC, BK_CHANNEL, BK_INFLOW, BK_COUETTE, BK_OPEN};
+constexpr int NGROUPS = 3 * 6;  // group = kind * 6 + index of the variant in BATCH_BCS
+
+// tiles per task: enough to hide the member lookup, few enough that a
+// 128^2 member still spreads over several blocks
+#ifndef BGK_TASK_TILES
+#define BGK_TASK_TILES 8
+#endif
+constexpr int TASK_TILES = BGK_TASK_TILES;
+
+// The planes are held as byte offsets from the member table's own device
+// address: a pointer read from memory is generic to the compiler and its
+// accesses become flat instructions (which also wait on the LDS counter),
+// while an offset from a kernel argument keeps them global loads and stores.
 struct Member {
-    void *set[2][8];  // the two population sets
-    void *r[2];       // the two pop-19 planes
+    long long set[2][8];  // the two population sets
+    long long r[2];       // the two pop-19 planes
     int l, m, off, store;
     long long rs;
     int bc;                  // BcKind
     int tiles_x;             // tiles per row of tiles
+    int ntiles;              // tiles of the member
     int tw_log2, th;         // tile shape: (1 << tw_log2) x th = 256 nodes
     int parity0, remaining;  // set holding the state at t = 0; steps left
     int rparity0;            // pop-19 plane holding the state at t = 0
     Phys<float> P;
 };
 
+// meta: [0, NGROUPS) task counts of the groups; [NGROUPS, 2 NGROUPS) their
+// first task in the table.  A task (member, first tile) covers the tiles
+// first .. min(first + TASK_TILES, ntiles) - 1.
 template <class S, int BC>
-__device__ __forceinline__ void member_node(const Lat &L, const Phys<float> &P, int i, int j)
+__global__ __launch_bounds__(256) void k_batch(const Member *__restrict__ M, const int2 *__restrict__ tasks,
+                                               const int *__restrict__ meta, int group, int t)
 {
-    fused_node<S, float, BC, false>(L, P, i, j);
-}
-
-template <class S>
-__device__ __forceinline__ void member_bc(const Lat &L, const Phys<float> &P, int bc, int i, int j)
-{
-    switch (bc) {
-    case BK_PERIODIC: member_node<S, BK_PERIODIC>(L, P, i, j); break;
-    case BK_CHANNEL: member_node<S, BK_CHANNEL>(L, P, i, j); break;
-    case BK_INFLOW: member_node<S, BK_INFLOW>(L, P, i, j); break;
-    case BK_COUETTE: member_node<S, BK_COUETTE>(L, P, i, j); break;
-    case BK_OPEN: member_node<S, BK_OPEN>(L, P, i, j); break;
-    default: member_node<S, BK_DRIVEN>(L, P, i, j); break;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_batch(const Member *__restrict__ M, const int2 *__restrict__ tiles,
-                                               const int *__restrict__ ntiles, int t)
-{
-    const int n = *ntiles;
+    const int n = meta[group];
+    const int2 *__restrict__ tk = tasks + meta[NGROUPS + group];
     for (int q = blockIdx.x; q < n; q += gridDim.x) {
-        const int2 tl = tiles[q];
+        const int2 tl = tk[q];
         const Member &d = M[tl.x];
         if (t >= d.remaining) continue;
         const int par = (d.parity0 + t) & 1;
-        const int tx = tl.y % d.tiles_x, ty = tl.y / d.tiles_x;
-        const int tw = 1 << d.tw_log2;
-        const int i = (tx << d.tw_log2) + ((int)threadIdx.x & (tw - 1)) + 1;
-        const int j = ty * d.th + ((int)threadIdx.x >> d.tw_log2) + 1;
-        if (i > d.l || j > d.m) continue;
+        const int rpar = (d.rparity0 + t) & 1;
+        char *const base = (char *)M;
         Lat L;
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            L.a[k] = d.set[par][k];
-            L.b[k] = d.set[par ^ 1][k];
+            L.a[k] = base + d.set[par][k];
+            L.b[k] = base + d.set[par ^ 1][k];
         }
-        const int rpar = (d.rparity0 + t) & 1;
-        L.r = d.r[rpar];
-        L.rn = d.r[rpar ^ 1];
+        L.r = base + d.r[rpar];
+        L.rn = base + d.r[rpar ^ 1];
         L.obs = nullptr;
         L.l = d.l;
         L.m = d.m;
@@ -91,10 +97,23 @@ __global__ __launch_bounds__(256) void k_batch(const Member *__restrict__ M, con
         L.jmax = 0;
         L.stepx = 0;
         L.stepy = 0;
-        switch (d.store) {
-        case MK_F16: member_bc<_Float16>(L, d.P, d.bc, i, j); break;
-        case MK_BF16: member_bc<bf16_t>(L, d.P, d.bc, i, j); break;
-        default: member_bc<float>(L, d.P, d.bc, i, j); break;
+        const Phys<float> P = d.P;
+        const int tiles_x = d.tiles_x, tw_log2 = d.tw_log2, th = d.th;
+        const int tend = min(tl.y + TASK_TILES, d.ntiles);
+        const int lane_x = (int)threadIdx.x & ((1 << tw_log2) - 1);
+        const int lane_y = (int)threadIdx.x >> tw_log2;
+        for (int tile = tl.y; tile < tend; tile++) {
+            const int tx = tile % tiles_x, ty = tile / tiles_x;
+            // column of this thread's node (16-bit kinds: first node of its pair)
+            const int col = (tx << tw_log2) + lane_x;
+            const int i = sizeof(S) == 2 ? 2 * col + 1 : col + 1;
+            const int j = ty * th + lane_y + 1;
+            if (i > L.l || j > L.m) continue;
+            // one node (fp32) or the pair (i, i+1) (16-bit kinds), as bgk_run's kernels
+            if constexpr (sizeof(S) == 2)
+                fused_pair<S, float, BC, false>(L, P, i, j);
+            else
+                fused_node<S, float, BC, false>(L, P, i, j);
         }
     }
 }

@@ -52,10 +52,11 @@ class Request:
         lx, ly = self.geometry()[:2]
         return lx * ly * self.steps
 
-    def params(self, device=0, form=1, rounding=1, tblock=1):
-        """bgk_params_t of the solve.  tblock=1 (one step per kernel) is the
-        step the batch kernel performs, so sequential and batched serving
-        run the same per-node code."""
+    def params(self, device=0, form=1, rounding=1, tblock=0):
+        """bgk_params_t of the solve.  tblock=0 lets a large member (one the
+        batch advances with bgk_run's own kernels) use the two-step kernel
+        where that is faster, as a plain solve does; every path gives the
+        same bits."""
         lx, ly, bc, svisc, u0, _ = self.geometry()
         narrow = self.prec in ("fp16", "bf16")
         return BgkParams(lx=lx, ly=ly, bc=BC[bc], impl=IMPL["fused"], prec=PREC[self.prec],

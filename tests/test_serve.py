@@ -38,7 +38,12 @@ def test_batched_members_equal_solves_alone():
             Request(2, "poiseuille", 256, 200.0, 960, "single", 0.0),
             Request(3, "cavity", 97, 100.0, 333, "fp16", 0.0),
             Request(4, "tgv", 300, 1000.0, 500, "bf16", 0.0),
-            Request(5, "cavity", 513, 1000.0, 250, "single", 0.0)]
+            Request(5, "cavity", 513, 1000.0, 250, "single", 0.0),
+            # members of >= 2^22 nodes advance with bgk_run's kernels (the
+            # two-step kernel for FP32) before the batch kernels
+            Request(6, "cavity", 2050, 1000.0, 37, "single", 0.0),
+            Request(7, "poiseuille", 4100, 500.0, 23, "single", 0.0),
+            Request(8, "tgv", 2048, 500.0, 30, "fp16", 0.0)]
     keep = {r.rid for r in reqs}
     bo, _ = serve.serve_batched(reqs, budget_bytes=60e9, chunk=10, check_every=100, keep=keep,
                                 tiles={"fp16": 3})
