@@ -1092,6 +1092,7 @@ struct bgk_batch {
     std::map<long long, hipGraphExec_t> graphs;  // key: (chunk, group, full grid)
     std::map<long long, int> uses;
     hipStream_t gs[bgk::NGROUPS];          // one stream per group, made on first use
+    int cus, grid_tb[6];
     hipEvent_t gev[bgk::NGROUPS];
     int grid;
     hipEvent_t ev0, ev1;
@@ -1140,21 +1141,51 @@ constexpr size_t SOLO_NODES = (size_t)1 << 22;
 constexpr size_t SMALL_GROUP = 512;
 bool member_solo(const bgk_ctx *c) { return (size_t)c->l * (size_t)c->m >= SOLO_NODES; }
 
-// entries of the batch task table for a member
-size_t member_tasks(const bgk_ctx *c, int tw_log2, int th)
+// An fp32 member advances two steps per launch (k_batch_tb) in a chunk of
+// even length that it runs to the end, unless its parameters switch the
+// two-step kernel off; otherwise (and for the 16-bit kinds, where the paired
+// one-step kernel is as fast) one step per launch.
+bool member_tb(const bgk_ctx *c, long long remaining, int chunk)
+{
+    return c->p.prec == BGK_PREC_SINGLE && c->p.tblock != 1 && (chunk & 1) == 0 && remaining >= chunk;
+}
+// two-step strip blocks of a member, h rows high
+int member_strips(const bgk_ctx *c) { return (c->l + bgk::TB_TW - 1) / bgk::TB_TW; }
+size_t member_tb_tiles(const bgk_ctx *c, int h)
+{
+    return (size_t)member_strips(c) * (size_t)((c->m + h - 1) / h);
+}
+// entries of the batch task table for a member: one per strip block for
+// the two-step kernel, one per TASK_TILES tiles otherwise
+size_t member_tasks(const bgk_ctx *c, int tw_log2, int th, bool tb, int h)
 {
     if (member_solo(c)) return 0;
+    if (tb) return member_tb_tiles(c, h);
     return (member_tiles(c, tw_log2, th) + bgk::TASK_TILES - 1) / bgk::TASK_TILES;
+}
+// Strip height of the two-step members of a chunk: the strip blocks of all
+// of them make whole rounds of the blocks the device holds (as k_fused_tb's
+// height does for one lattice), in as few rounds as strips of at most
+// TB_BATCH_H rows allow, and at least TB_HMIN rows.
+int batch_tb_h(size_t strip_rows, int slots)
+{
+    if (slots < 1) slots = 1;
+    const size_t per_round = (size_t)slots * bgk::TB_BATCH_H;
+    const size_t rounds = strip_rows > per_round ? (strip_rows + per_round - 1) / per_round : 1;
+    size_t h = (strip_rows + rounds * slots - 1) / (rounds * slots);
+    if (h < (size_t)TB_HMIN) h = TB_HMIN;
+    if (h > (size_t)bgk::TB_BATCH_H) h = bgk::TB_BATCH_H;
+    return (int)h;
 }
 int member_kind(const bgk_ctx *c)
 {
     return c->p.prec == BGK_PREC_FP16 ? bgk::MK_F16 : (c->p.prec == BGK_PREC_BF16 ? bgk::MK_BF16 : bgk::MK_F32);
 }
-int member_group(const bgk_ctx *c)
+int member_group(const bgk_ctx *c, bool tb)
 {
     int v = 0;
     while (v < 5 && bgk::BATCH_BCS[v] != c->p.bc) v++;
-    return member_kind(c) * 6 + v;
+    return (tb ? bgk::MK_F32TB : member_kind(c)) * 6 + v;
 }
 
 // one step of a group's members
@@ -1176,13 +1207,50 @@ void launch_batch_s(bgk_batch_t *b, int g, int nb, int t)
     default: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_DRIVEN>), grid, blk, 0, s, dm, dt, meta, g, t); break;
     }
 }
+void launch_batch_tb(bgk_batch_t *b, int g, int nb, int t2)
+{
+    const dim3 grid(nb), blk(256);
+    const bgk::Member *dm = b->d_members;
+    const int2 *dt = b->d_tiles;
+    const int *meta = b->d_ntiles;
+    hipStream_t s = b->gs[g];
+    switch (bgk::BATCH_BCS[g % 6]) {
+    case bgk::BK_PERIODIC: hipLaunchKernelGGL((bgk::k_batch_tb<bgk::BK_PERIODIC>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
+    case bgk::BK_CHANNEL: hipLaunchKernelGGL((bgk::k_batch_tb<bgk::BK_CHANNEL>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
+    case bgk::BK_INFLOW: hipLaunchKernelGGL((bgk::k_batch_tb<bgk::BK_INFLOW>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
+    case bgk::BK_COUETTE: hipLaunchKernelGGL((bgk::k_batch_tb<bgk::BK_COUETTE>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
+    case bgk::BK_OPEN: hipLaunchKernelGGL((bgk::k_batch_tb<bgk::BK_OPEN>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
+    default: hipLaunchKernelGGL((bgk::k_batch_tb<bgk::BK_DRIVEN>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
+    }
+}
+// launch t of a group's chain (a two-step group's chain has chunk / 2)
 void launch_batch(bgk_batch_t *b, int g, int nb, int t)
 {
     switch (g / 6) {
     case bgk::MK_F16: launch_batch_s<_Float16>(b, g, nb, t); break;
     case bgk::MK_BF16: launch_batch_s<bgk::bf16_t>(b, g, nb, t); break;
+    case bgk::MK_F32TB: launch_batch_tb(b, g, nb, t); break;
     default: launch_batch_s<float>(b, g, nb, t); break;
     }
+}
+// blocks of the two-step batch kernel the device holds at once: its grid,
+// since a persistent block that starts late would run its whole share late
+int grid_tb(bgk_batch_t *b, int v)
+{
+    if (b->grid_tb[v] > 0) return b->grid_tb[v];
+    int per_cu = 0;
+    hipError_t e;
+    switch (bgk::BATCH_BCS[v]) {
+    case bgk::BK_PERIODIC: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bgk::k_batch_tb<bgk::BK_PERIODIC>, 256, 0); break;
+    case bgk::BK_CHANNEL: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bgk::k_batch_tb<bgk::BK_CHANNEL>, 256, 0); break;
+    case bgk::BK_INFLOW: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bgk::k_batch_tb<bgk::BK_INFLOW>, 256, 0); break;
+    case bgk::BK_COUETTE: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bgk::k_batch_tb<bgk::BK_COUETTE>, 256, 0); break;
+    case bgk::BK_OPEN: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bgk::k_batch_tb<bgk::BK_OPEN>, 256, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bgk::k_batch_tb<bgk::BK_DRIVEN>, 256, 0); break;
+    }
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    b->grid_tb[v] = per_cu * b->cus;
+    return b->grid_tb[v];
 }
 }  // extern "C++"
 
@@ -1241,6 +1309,8 @@ int bgk_batch_create(int32_t device, double budget_bytes, int32_t max_members, b
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
         cus = 256;
     b->grid = cus * 8;
+    b->cus = cus;
+    for (int v = 0; v < 6; v++) b->grid_tb[v] = 0;
     for (int g = 0; g < bgk::NGROUPS; g++) {
         b->gs[g] = nullptr;
         b->gev[g] = nullptr;
@@ -1373,13 +1443,26 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
     int nm = 0;
     size_t nt = 0;
     std::vector<int> members_slot;
+    std::vector<char> tbm;  // member advances two steps per launch in this chunk
+    size_t strip_rows = 0;
+    int tbv = -1;  // a variant of the two-step members
     for (size_t q = 0; q < b->slots.size(); q++) {
         const auto &sl = b->slots[q];
         if (!sl.ctx || sl.remaining <= 0) continue;
         const bgk_ctx *c = sl.ctx;
-        nt += member_tasks(c, sl.tw_log2, sl.th);
+        const bool tb = member_tb(c, sl.remaining, chunk) && !member_solo(c);
+        if (tb) {
+            strip_rows += (size_t)member_strips(c) * (size_t)c->m;
+            tbv = member_group(c, true) % 6;
+        }
         members_slot.push_back((int)q);
+        tbm.push_back(tb);
         nm++;
+    }
+    const int tbh = tbv >= 0 ? batch_tb_h(strip_rows, grid_tb(b, tbv)) : bgk::TB_BATCH_H;
+    for (int mi = 0; mi < nm; mi++) {
+        const auto &sl = b->slots[members_slot[mi]];
+        nt += member_tasks(sl.ctx, sl.tw_log2, sl.th, tbm[mi], tbh);
     }
     if (finished) *finished = 0;
     if (nm == 0) {
@@ -1393,7 +1476,7 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
     size_t cnt[G] = {}, pos[G];
     for (int mi = 0; mi < nm; mi++) {
         const auto &sl = b->slots[members_slot[mi]];
-        cnt[member_group(sl.ctx)] += member_tasks(sl.ctx, sl.tw_log2, sl.th);
+        cnt[member_group(sl.ctx, tbm[mi])] += member_tasks(sl.ctx, sl.tw_log2, sl.th, tbm[mi], tbh);
     }
     for (int g = 0, acc = 0; g < G; g++) {
         pos[g] = (size_t)acc;
@@ -1420,16 +1503,23 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
         d.rs = c->pitch;
         d.store = member_kind(c);
         d.bc = c->p.bc;  // BGK_BC_* and BcKind share the numbering of the foldable variants
-        d.tiles_x = member_tiles_x(c, sl.tw_log2);
         d.tw_log2 = sl.tw_log2;
-        d.th = sl.th;
         d.parity0 = c->cur;
         d.remaining = (int)(sl.remaining < chunk ? sl.remaining : chunk);
         d.P = phys<float>(c);
-        d.ntiles = (int)member_tiles(c, sl.tw_log2, sl.th);
+        if (tbm[mi]) {
+            d.tiles_x = member_strips(c);
+            d.th = tbh;
+            d.ntiles = (int)member_tb_tiles(c, tbh);
+        } else {
+            d.tiles_x = member_tiles_x(c, sl.tw_log2);
+            d.th = sl.th;
+            d.ntiles = (int)member_tiles(c, sl.tw_log2, sl.th);
+        }
         if (member_solo(c)) continue;
-        size_t &t = pos[member_group(c)];
-        for (int q = 0; q < d.ntiles; q += bgk::TASK_TILES) b->h_tiles[t++] = make_int2(mi, q);
+        size_t &t = pos[member_group(c, tbm[mi])];
+        const int per = tbm[mi] ? 1 : bgk::TASK_TILES;
+        for (int q = 0; q < d.ntiles; q += per) b->h_tiles[t++] = make_int2(mi, q);
     }
     BHIPCHK(b, hipMemcpyAsync(b->d_members, b->h_members, sizeof(bgk::Member) * nm, hipMemcpyHostToDevice, b->s));
     BHIPCHK(b, hipMemcpyAsync(b->d_tiles, b->h_tiles, sizeof(int2) * nt, hipMemcpyHostToDevice, b->s));
@@ -1459,21 +1549,26 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
         }
         // a group with few tasks launches fewer blocks, leaving the rest of
         // the chip to the other groups' chains
-        const int nb = cnt[g] <= SMALL_GROUP ? SMALL_GROUP : b->grid;
-        const long long key = ((long long)chunk * G + g) * 2 + (nb == b->grid);
+        // (two-step groups: always the full grid, whose surplus blocks exit
+        // at once; their strip height already sizes the work to it)
+        const bool two = g / 6 == bgk::MK_F32TB;
+        const int full = two ? grid_tb(b, g % 6) : b->grid;
+        const int nb = !two && cnt[g] <= SMALL_GROUP ? (int)SMALL_GROUP : full;
+        const int nl = two ? chunk / 2 : chunk;  // launches of the chain
+        const long long key = ((long long)chunk * G + g) * 2 + (nb == full);
         BHIPCHK(b, hipStreamWaitEvent(b->gs[g], prev, 0));
         // a chain seen for the first time is launched directly; its graph
         // is captured when it comes again (capture and instantiation cost
         // more than the launches of one chunk)
         if (++b->uses[key] < 2) {
-            for (int tt = 0; tt < chunk; tt++) launch_batch(b, g, nb, tt);
+            for (int tt = 0; tt < nl; tt++) launch_batch(b, g, nb, tt);
             BHIPCHK(b, hipGetLastError());
         } else {
             hipGraphExec_t ge;
             auto it = b->graphs.find(key);
             if (it == b->graphs.end()) {
                 BHIPCHK(b, hipStreamBeginCapture(b->gs[g], hipStreamCaptureModeThreadLocal));
-                for (int tt = 0; tt < chunk; tt++) launch_batch(b, g, nb, tt);
+                for (int tt = 0; tt < nl; tt++) launch_batch(b, g, nb, tt);
                 hipGraph_t gr = nullptr;
                 BHIPCHK(b, hipStreamEndCapture(b->gs[g], &gr));
                 hipError_t e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
@@ -1498,8 +1593,10 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
         auto &sl = b->slots[members_slot[mi]];
         const int steps = b->h_members[mi].remaining;
         if (!member_solo(sl.ctx)) {
-            sl.ctx->cur = (sl.ctx->cur + steps) & 1;
-            sl.ctx->rcur = (sl.ctx->rcur + steps) & 1;
+            // each launch swaps the sets and the pop-19 planes once
+            const int launches = tbm[mi] ? steps / 2 : steps;
+            sl.ctx->cur = (sl.ctx->cur + launches) & 1;
+            sl.ctx->rcur = (sl.ctx->rcur + launches) & 1;
         }
         sl.remaining -= steps;
         sl.done += steps;

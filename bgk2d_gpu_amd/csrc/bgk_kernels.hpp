@@ -737,8 +737,9 @@ __device__ __forceinline__ void tb_load(const Lat &L, int i, int j, R xr[9])
     xr[8] = *(const R *)(pull_base<S>(L, L.r, 8) + bo);
 }
 
+// the strip block (bx, by): columns bx*TB_TW+1.., rows by*H+1..
 template <class S, class C, int BC, bool OBS>
-__global__ __launch_bounds__(256) void k_fused_tb(Lat L, Phys<C> P, int H)
+__device__ __forceinline__ void tb_block(const Lat &L, const Phys<C> &P, int H, int bx, int by)
 {
     using R = typename CodecT<S, C>::raw;
     constexpr int TW = TB_TW, E = TW + 2;
@@ -747,7 +748,7 @@ __global__ __launch_bounds__(256) void k_fused_tb(Lat L, Phys<C> P, int H)
     constexpr int RB = TB_RB;
     __shared__ R rim[2][RB][3];
     const int l = L.l, m = L.m;
-    const int i0 = blockIdx.x * TW + 1, j0 = blockIdx.y * H + 1;
+    const int i0 = bx * TW + 1, j0 = by * H + 1;
     const int ncols = min(TW, l - i0 + 1), jend = min(j0 + H - 1, m);
     const int tid = threadIdx.x;
     // node row / column that a ring position holds, -1 for a halo of a
@@ -834,6 +835,12 @@ __global__ __launch_bounds__(256) void k_fused_tb(Lat L, Phys<C> P, int H)
         for (int k = 0; k < 8; k++) st_nt((R *)(pull_base<S>(L, L.b[k], 8) + bo), encode<S, C>(P, o[k], i, q, k));
         st_nt((R *)(pull_base<S>(L, L.rn, 8) + bo), encode<S, C>(P, o[8], i, q, 8));
     }
+}
+
+template <class S, class C, int BC, bool OBS>
+__global__ __launch_bounds__(256) void k_fused_tb(Lat L, Phys<C> P, int H)
+{
+    tb_block<S, C, BC, OBS>(L, P, H, blockIdx.x, blockIdx.y);
 }
 
 // ------------------------------------------------------------ diagnostics

@@ -34,7 +34,12 @@ enum MemberKind {
 
 // boundary variants the batch kernels serve, in group order
 constexpr int BATCH_BCS[6] = {BK_DRIVEN, BK_PERIODIC, BK_CHANNEL, BK_INFLOW, BK_COUETTE, BK_OPEN};
-constexpr int NGROUPS = 3 * 6;  // group = kind * 6 + index of the variant in BATCH_BCS
+// group = kind * 6 + index of the variant in BATCH_BCS; kind MK_F32TB holds
+// the fp32 members that advance two steps per launch in this chunk
+constexpr int MK_F32TB = 3;
+constexpr int NGROUPS = 4 * 6;
+// largest strip height of the two-step batch kernel
+constexpr int TB_BATCH_H = 64;
 
 // tiles per task: enough to hide the member lookup, few enough that a
 // 128^2 member still spreads over several blocks
@@ -53,9 +58,9 @@ struct Member {
     int l, m, off, store;
     long long rs;
     int bc;                  // BcKind
-    int tiles_x;             // tiles per row of tiles
+    int tiles_x;             // tiles per row of tiles (strips for the two-step kernel)
     int ntiles;              // tiles of the member
-    int tw_log2, th;         // tile shape: (1 << tw_log2) x th = 256 nodes
+    int tw_log2, th;         // tile shape: (1 << tw_log2) x th = 256 nodes (two-step: th = strip height)
     int parity0, remaining;  // set holding the state at t = 0; steps left
     int rparity0;            // pop-19 plane holding the state at t = 0
     Phys<float> P;
@@ -115,6 +120,84 @@ __global__ __launch_bounds__(256) void k_batch(const Member *__restrict__ M, con
             else
                 fused_node<S, float, BC, false>(L, P, i, j);
         }
+    }
+}
+
+// Two FUSED steps of every member of an MK_F32TB group per launch: a task
+// is one strip block of k_fused_tb (TB_TW columns by th rows), run by the
+// same device code, so the values are those of two one-step launches.
+// Launch t2 performs steps 2 t2 and 2 t2 + 1 of the chunk; a member in the
+// group has at least 2 (t2 + 1) steps left.  Every task begins with a
+// barrier: the previous task's last reads of the LDS ring must be done
+// before this one refills it.
+// The tables are read through the constant address space: they do not
+// change during a launch, and so their loads stay scalar after the
+// barriers (which would otherwise make them vector loads, the members'
+// fields then held in VGPRs).  Field by field, so that only what is used is
+// loaded and nothing goes through private memory.
+template <class T>
+__device__ __forceinline__ T cld(const T *p)
+{
+    return *(const __attribute__((address_space(4))) T *)p;
+}
+__device__ __forceinline__ Phys<float> cld_phys(const Phys<float> *p)
+{
+    static_assert(sizeof(Phys<float>) == 15 * 4, "cld_phys copies every field of Phys");
+    Phys<float> r;
+    r.omega = cld(&p->omega);
+    r.fgrad = cld(&p->fgrad);
+    r.force = cld(&p->force);
+    r.sc = cld(&p->sc);
+    r.inv = cld(&p->inv);
+#pragma unroll
+    for (int k = 0; k < 3; k++) r.inflow[k] = cld(&p->inflow[k]);
+    r.umax = cld(&p->umax);
+    r.channel_force = cld(&p->channel_force);
+    r.form = cld(&p->form);
+    r.rounding = cld(&p->rounding);
+    r.les = cld(&p->les);
+    r.trick1 = cld(&p->trick1);
+    r.ctes = cld(&p->ctes);
+    return r;
+}
+
+template <int BC>
+__global__ __launch_bounds__(256) void k_batch_tb(const Member *__restrict__ M, const int2 *__restrict__ tasks,
+                                                  const int *__restrict__ meta, int group, int t2)
+{
+    const int n = cld(meta + group);
+    const int *tk = (const int *)tasks + 2 * cld(meta + NGROUPS + group);
+    for (int q = blockIdx.x; q < n; q += gridDim.x) {
+        __syncthreads();
+        const int mi = cld(tk + 2 * q), tile = cld(tk + 2 * q + 1);
+        const Member *d = M + mi;
+        if (2 * t2 + 2 > cld(&d->remaining)) continue;
+        const int par = (cld(&d->parity0) + t2) & 1;
+        const int rpar = (cld(&d->rparity0) + t2) & 1;
+        char *const base = (char *)M;
+        Lat L;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            L.a[k] = base + cld(&d->set[par][k]);
+            L.b[k] = base + cld(&d->set[par ^ 1][k]);
+        }
+        L.r = base + cld(&d->r[rpar]);
+        L.rn = base + cld(&d->r[rpar ^ 1]);
+        L.obs = nullptr;
+        L.l = cld(&d->l);
+        L.m = cld(&d->m);
+        L.pitch = 0;
+        L.off = cld(&d->off);
+        L.rs = cld(&d->rs);
+        L.imin = L.l;
+        L.imax = 0;
+        L.jmin = L.m;
+        L.jmax = 0;
+        L.stepx = 0;
+        L.stepy = 0;
+        const Phys<float> P = cld_phys(&d->P);
+        const int tiles_x = cld(&d->tiles_x);
+        tb_block<float, float, BC, false>(L, P, cld(&d->th), tile % tiles_x, tile / tiles_x);
     }
 }
 
