@@ -352,8 +352,10 @@ int launch_tb_t(bgk_ctx *c)
     }
     Lat L = lat(c);
     Phys<C> P = phys<C>(c);
-    const dim3 g((c->l + TB_TW - 1) / TB_TW, (c->m + c->tb_h - 1) / c->tb_h);
-    hipLaunchKernelGGL((k_fused_tb<S, C, BC, OBS>), g, dim3(256), 0, c->s, L, P, c->tb_h);
+    const int strips = (c->l + TB_TW - 1) / TB_TW, rows = (c->m + c->tb_h - 1) / c->tb_h;
+    const int rows8 = (rows + NXCD - 1) / NXCD * NXCD;
+    hipLaunchKernelGGL((k_fused_tb<S, C, BC, OBS>), dim3(strips * rows8), dim3(256), 0, c->s, L, P, c->tb_h, strips,
+                       rows);
     c->cur = 1 - c->cur;
     c->rcur = 1 - c->rcur;
     HIPCHK(c, hipGetLastError());

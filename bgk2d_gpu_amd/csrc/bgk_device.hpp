@@ -42,7 +42,12 @@ struct bf16_t {
 // the rounding unbiased; with round-to-nearest, slowly varying fields lose
 // their sub-ulp per-step changes systematically.  The offset is a hash of
 // the value's own bits and its position (i, j, population), so results are
-// deterministic and independent of launch grouping.
+// deterministic and independent of launch grouping.  The conversion itself
+// is gfx950's v_cvt_sr_f16_f32 / v_cvt_sr_bf16_f32: it adds the top 13
+// (FP16) or 16 (BF16) bits of its random operand to the magnitude below the
+// kept bits (aligned to the top of the dropped bits for half subnormals)
+// and truncates; NaN stays NaN, overflow gives inf.  One instruction in
+// place of the range checks, add, mask and conversion.
 // position salt: 24-bit multiplies (full rate on CDNA) and a Weyl step per
 // population; the per-store hash below does the mixing
 __host__ __device__ inline uint32_t mul24(uint32_t a, uint32_t b)
@@ -58,12 +63,16 @@ __host__ __device__ inline uint32_t salt_of(int i, int j, int k)
     return mul24((uint32_t)i, 0x9E3779u) + mul24((uint32_t)j, 0x85EBCAu) + (uint32_t)k * 0x9E3779B9u;
 }
 // per-store offset: one 24-bit multiply (full rate on CDNA) of the low bits
-// of value^salt, which change from step to step, then an xorshift
+// of value^salt, which change from step to step, then an xorshift; bits
+// 8.. of the result are the offset
 __host__ __device__ inline uint32_t sr_hash(uint32_t bits, uint32_t salt)
 {
     uint32_t h = mul24(bits ^ salt, 0xB5297Au);
     return h ^ (h >> 15);
 }
+
+typedef __fp16 sr_half2 __attribute__((ext_vector_type(2)));
+typedef short sr_short2 __attribute__((ext_vector_type(2)));
 
 template <class S, class C>
 struct Codec;
@@ -93,13 +102,11 @@ struct Codec<_Float16, float> {
     __device__ __forceinline__ static float dec(_Float16 v, float inv) { return (float)v * inv; }
     __device__ __forceinline__ static _Float16 enc(float v, float sc, int sr, uint32_t salt)
     {
-        float x = v * sc;
+        const float x = v * sc;
         if (sr) {
-            uint32_t b = __float_as_uint(x), a = b & 0x7fffffffu;
-            if (a >= 0x38800000u && a < 0x477fe000u) {  // normal half range
-                b += (sr_hash(b, salt) >> 8) & 0x1fffu;
-                x = __uint_as_float(b & ~0x1fffu);
-            }
+            // offset bits 8..20 of the hash as the top 13 bits of the operand
+            const sr_half2 h = __builtin_amdgcn_cvt_sr_f16_f32(sr_half2{}, x, sr_hash(__float_as_uint(x), salt) << 11, false);
+            return (_Float16)h[0];
         }
         return (_Float16)x;
     }
@@ -113,13 +120,15 @@ struct Codec<bf16_t, float> {
     }
     __device__ __forceinline__ static bf16_t enc(float v, float sc, int sr, uint32_t salt)
     {
-        uint32_t x = __float_as_uint(v * sc);
+        const float f = v * sc;
+        uint32_t x = __float_as_uint(f);
         bf16_t r;
-        if ((x & 0x7fffffffu) > 0x7f800000u) {
+        if (sr) {
+            // offset bits 8..23 of the hash as the top 16 bits of the operand
+            const sr_short2 h = __builtin_amdgcn_cvt_sr_bf16_f32(sr_short2{}, f, sr_hash(x, salt) << 8, false);
+            r.bits = (uint16_t)h[0];
+        } else if ((x & 0x7fffffffu) > 0x7f800000u) {
             r.bits = (uint16_t)((x >> 16) | 0x40u);
-        } else if (sr && (x & 0x7fffffffu) < 0x7f7f0000u) {
-            x += (sr_hash(x, salt) >> 8) & 0xffffu;
-            r.bits = (uint16_t)(x >> 16);
         } else {
             x += 0x7fffu + ((x >> 16) & 1u);
             r.bits = (uint16_t)(x >> 16);

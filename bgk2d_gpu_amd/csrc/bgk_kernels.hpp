@@ -837,10 +837,27 @@ __device__ __forceinline__ void tb_block(const Lat &L, const Phys<C> &P, int H, 
     }
 }
 
-template <class S, class C, int BC, bool OBS>
-__global__ __launch_bounds__(256) void k_fused_tb(Lat L, Phys<C> P, int H)
+// Blocks are dispatched to the eight XCDs (each with its own L2) in turn:
+// block b runs on XCD b % 8.  tb_xcd_block puts every strip of a row of
+// strip blocks on one XCD, so the lines that neighbouring strips both read
+// (the rim columns, and the x-shifted pulls across the strip edge) meet in
+// one L2.  The grid covers rows rounded up to a multiple of eight; blocks
+// past the last row return at once.
+constexpr int NXCD = 8;
+__device__ __forceinline__ bool tb_xcd_block(int b, int strips, int rows, int &bx, int &by)
 {
-    tb_block<S, C, BC, OBS>(L, P, H, blockIdx.x, blockIdx.y);
+    const int x = b % NXCD, k = b / NXCD;
+    bx = k % strips;
+    by = NXCD * (k / strips) + x;
+    return by < rows;
+}
+
+template <class S, class C, int BC, bool OBS>
+__global__ __launch_bounds__(256) void k_fused_tb(Lat L, Phys<C> P, int H, int strips, int rows)
+{
+    int bx, by;
+    if (!tb_xcd_block((int)blockIdx.x, strips, rows, bx, by)) return;
+    tb_block<S, C, BC, OBS>(L, P, H, bx, by);
 }
 
 // ------------------------------------------------------------ diagnostics

@@ -119,8 +119,11 @@ float orc_bf16_to_f32(uint16_t v)
 }
 
 /* Stochastic rounding of half/bf16 stores: a hash of the value's low bits
- * and its position (i, j, k) is added below the kept bits before
- * truncation (the same rule as the HIP codec).                          */
+ * and its position (i, j, k) gives a random word whose top bits are added
+ * below the kept bits of the magnitude before truncation, the rule of
+ * gfx950's v_cvt_sr_f16_f32 / v_cvt_sr_bf16_f32 that the HIP codec uses
+ * (restated from the hardware's observed behaviour, pinned against the
+ * probe table tests/golden/sr_convert.npz).                             */
 static uint32_t salt_of(int i, int j, int k)
 {
     return ((uint32_t)i & 0xffffffu) * 0x9E3779u + ((uint32_t)j & 0xffffffu) * 0x85EBCAu +
@@ -131,29 +134,59 @@ static uint32_t sr_hash(uint32_t bits, uint32_t salt)
     uint32_t h = ((bits ^ salt) & 0xffffffu) * 0xB5297Au;
     return h ^ (h >> 15);
 }
+/* binary32 bits b to binary16 with random word r: the top 13 bits of r
+ * are added to the 24-bit significand below the 11 kept bits (for a half
+ * subnormal result, aligned to the top of the wider dropped field), then
+ * truncated; NaN keeps its top payload bits and turns quiet, a carry past
+ * 65504 gives inf, binary32 zeros and subnormals give signed zero.       */
+uint16_t orc_sr_f16(uint32_t b, uint32_t r)
+{
+    uint32_t sign = (b >> 16) & 0x8000u, a = b & 0x7fffffffu;
+    if (a >= 0x7f800000u)
+        return (uint16_t)(sign | 0x7c00u | (a > 0x7f800000u ? (0x200u | ((a >> 13) & 0x3ffu)) : 0u));
+    if (a < 0x00800000u) return (uint16_t)sign;
+    int e = (int)(a >> 23) - 127;
+    uint64_t full = (a & 0x7fffffu) | 0x800000u;
+    int d = e >= -14 ? 13 : 13 + (-14 - e); /* dropped significand bits */
+    if (d >= 37) return (uint16_t)sign;    /* full + offset < 2^d */
+    uint32_t v = (uint32_t)((full + ((uint64_t)(r >> 19) << (d - 13))) >> d);
+    if (e < -14) return (uint16_t)(sign | v); /* subnormal (0x400: carried into 2^-14) */
+    if (v >= 0x800u) {
+        v >>= 1;
+        e++;
+    }
+    if (e > 15) return (uint16_t)(sign | 0x7c00u);
+    return (uint16_t)(sign | ((uint32_t)(e + 15) << 10) | (v & 0x3ffu));
+}
+/* binary32 bits b to bfloat16 with random word r: the top 16 bits of r are
+ * added to the magnitude, then truncated; NaN turns quiet, a carry into
+ * the exponent's all-ones pattern gives inf.                             */
+uint16_t orc_sr_bf16(uint32_t b, uint32_t r)
+{
+    uint32_t a = b & 0x7fffffffu;
+    if (a > 0x7f800000u) return (uint16_t)((b >> 16) | 0x40u);
+    if (a == 0x7f800000u) return (uint16_t)(b >> 16);
+    uint32_t a2 = a + (r >> 16);
+    if (a2 >= 0x7f800000u) return (uint16_t)(((b >> 16) & 0x8000u) | 0x7f80u);
+    return (uint16_t)(((b & 0x80000000u) | a2) >> 16);
+}
 static uint16_t enc_f16(float v, float sc, int sr, int i, int j, int k)
 {
     float x = v * sc;
     if (sr) {
         uint32_t b;
         memcpy(&b, &x, 4);
-        uint32_t a = b & 0x7fffffffu;
-        if (a >= 0x38800000u && a < 0x477fe000u) {
-            b += (sr_hash(b, salt_of(i, j, k)) >> 8) & 0x1fffu;
-            b &= ~0x1fffu;
-            memcpy(&x, &b, 4);
-        }
+        return orc_sr_f16(b, sr_hash(b, salt_of(i, j, k)) << 11);
     }
     return orc_f32_to_f16(x);
 }
 static uint16_t enc_bf16(float v, float sc, int sr, int i, int j, int k)
 {
     float x = v * sc;
-    uint32_t b;
-    memcpy(&b, &x, 4);
-    if (sr && (b & 0x7fffffffu) < 0x7f7f0000u) {
-        b += (sr_hash(b, salt_of(i, j, k)) >> 8) & 0xffffu;
-        return (uint16_t)(b >> 16);
+    if (sr) {
+        uint32_t b;
+        memcpy(&b, &x, 4);
+        return orc_sr_bf16(b, sr_hash(b, salt_of(i, j, k)) << 8);
     }
     return orc_f32_to_bf16(x);
 }

@@ -97,6 +97,9 @@ uint16_t orc_f32_to_f16(float f);
 float    orc_f16_to_f32(uint16_t h);
 uint16_t orc_f32_to_bf16(float f);
 float    orc_bf16_to_f32(uint16_t h);
+/* stochastic-rounding conversions of binary32 bits b with random word r  */
+uint16_t orc_sr_f16(uint32_t b, uint32_t r);
+uint16_t orc_sr_bf16(uint32_t b, uint32_t r);
 
 #ifdef __cplusplus
 }

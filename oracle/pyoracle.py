@@ -85,6 +85,9 @@ def lib():
         L.orc_f32_to_bf16.restype = C.c_uint16
         L.orc_bf16_to_f32.argtypes = [C.c_uint16]
         L.orc_bf16_to_f32.restype = C.c_float
+        for f in (L.orc_sr_f16, L.orc_sr_bf16):
+            f.argtypes = [C.c_uint32, C.c_uint32]
+            f.restype = C.c_uint16
         _lib = L
     return _lib
 
