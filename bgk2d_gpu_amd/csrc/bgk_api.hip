@@ -1094,7 +1094,7 @@ struct bgk_batch {
     std::map<long long, hipGraphExec_t> graphs;  // key: (chunk, group, full grid)
     std::map<long long, int> uses;
     hipStream_t gs[bgk::NGROUPS];          // one stream per group, made on first use
-    int cus, grid_tb[6];
+    int cus, grid_tb[3][6];
     hipEvent_t gev[bgk::NGROUPS];
     int grid;
     hipEvent_t ev0, ev1;
@@ -1143,13 +1143,15 @@ constexpr size_t SOLO_NODES = (size_t)1 << 22;
 constexpr size_t SMALL_GROUP = 512;
 bool member_solo(const bgk_ctx *c) { return (size_t)c->l * (size_t)c->m >= SOLO_NODES; }
 
-// An fp32 member advances two steps per launch (k_batch_tb) in a chunk of
-// even length that it runs to the end, unless its parameters switch the
-// two-step kernel off; otherwise (and for the 16-bit kinds, where the paired
-// one-step kernel is as fast) one step per launch.
+// A member advances two steps per launch (k_batch_tb) in a chunk of even
+// length that it runs to the end, with bgk_run's rule for the kernel:
+// tblock = 2 always, tblock = 0 for fp32 storage (for the 16-bit kinds the
+// one-step groups, which overlap, served a mixed stream 5-10 % faster),
+// tblock = 1 never; otherwise one step per launch.
 bool member_tb(const bgk_ctx *c, long long remaining, int chunk)
 {
-    return c->p.prec == BGK_PREC_SINGLE && c->p.tblock != 1 && (chunk & 1) == 0 && remaining >= chunk;
+    const bool two = c->p.tblock == 2 || (c->p.tblock == 0 && c->p.prec == BGK_PREC_SINGLE);
+    return two && (chunk & 1) == 0 && remaining >= chunk;
 }
 // two-step strip blocks of a member, h rows high
 int member_strips(const bgk_ctx *c) { return (c->l + bgk::TB_TW - 1) / bgk::TB_TW; }
@@ -1187,7 +1189,7 @@ int member_group(const bgk_ctx *c, bool tb)
 {
     int v = 0;
     while (v < 5 && bgk::BATCH_BCS[v] != c->p.bc) v++;
-    return (tb ? bgk::MK_F32TB : member_kind(c)) * 6 + v;
+    return ((tb ? bgk::TWO_STEP : 0) + member_kind(c)) * 6 + v;
 }
 
 // one step of a group's members
@@ -1209,6 +1211,7 @@ void launch_batch_s(bgk_batch_t *b, int g, int nb, int t)
     default: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_DRIVEN>), grid, blk, 0, s, dm, dt, meta, g, t); break;
     }
 }
+template <class S>
 void launch_batch_tb(bgk_batch_t *b, int g, int nb, int t2)
 {
     const dim3 grid(nb), blk(256);
@@ -1217,12 +1220,12 @@ void launch_batch_tb(bgk_batch_t *b, int g, int nb, int t2)
     const int *meta = b->d_ntiles;
     hipStream_t s = b->gs[g];
     switch (bgk::BATCH_BCS[g % 6]) {
-    case bgk::BK_PERIODIC: hipLaunchKernelGGL((bgk::k_batch_tb<bgk::BK_PERIODIC>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
-    case bgk::BK_CHANNEL: hipLaunchKernelGGL((bgk::k_batch_tb<bgk::BK_CHANNEL>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
-    case bgk::BK_INFLOW: hipLaunchKernelGGL((bgk::k_batch_tb<bgk::BK_INFLOW>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
-    case bgk::BK_COUETTE: hipLaunchKernelGGL((bgk::k_batch_tb<bgk::BK_COUETTE>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
-    case bgk::BK_OPEN: hipLaunchKernelGGL((bgk::k_batch_tb<bgk::BK_OPEN>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
-    default: hipLaunchKernelGGL((bgk::k_batch_tb<bgk::BK_DRIVEN>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
+    case bgk::BK_PERIODIC: hipLaunchKernelGGL((bgk::k_batch_tb<S, bgk::BK_PERIODIC>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
+    case bgk::BK_CHANNEL: hipLaunchKernelGGL((bgk::k_batch_tb<S, bgk::BK_CHANNEL>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
+    case bgk::BK_INFLOW: hipLaunchKernelGGL((bgk::k_batch_tb<S, bgk::BK_INFLOW>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
+    case bgk::BK_COUETTE: hipLaunchKernelGGL((bgk::k_batch_tb<S, bgk::BK_COUETTE>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
+    case bgk::BK_OPEN: hipLaunchKernelGGL((bgk::k_batch_tb<S, bgk::BK_OPEN>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
+    default: hipLaunchKernelGGL((bgk::k_batch_tb<S, bgk::BK_DRIVEN>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
     }
 }
 // launch t of a group's chain (a two-step group's chain has chunk / 2)
@@ -1231,28 +1234,39 @@ void launch_batch(bgk_batch_t *b, int g, int nb, int t)
     switch (g / 6) {
     case bgk::MK_F16: launch_batch_s<_Float16>(b, g, nb, t); break;
     case bgk::MK_BF16: launch_batch_s<bgk::bf16_t>(b, g, nb, t); break;
-    case bgk::MK_F32TB: launch_batch_tb(b, g, nb, t); break;
+    case bgk::TWO_STEP + bgk::MK_F32: launch_batch_tb<float>(b, g, nb, t); break;
+    case bgk::TWO_STEP + bgk::MK_F16: launch_batch_tb<_Float16>(b, g, nb, t); break;
+    case bgk::TWO_STEP + bgk::MK_BF16: launch_batch_tb<bgk::bf16_t>(b, g, nb, t); break;
     default: launch_batch_s<float>(b, g, nb, t); break;
     }
 }
 // blocks of the two-step batch kernel the device holds at once: its grid,
 // since a persistent block that starts late would run its whole share late
-int grid_tb(bgk_batch_t *b, int v)
+template <class S>
+int occupancy_tb(int v, int *per_cu)
 {
-    if (b->grid_tb[v] > 0) return b->grid_tb[v];
-    int per_cu = 0;
-    hipError_t e;
     switch (bgk::BATCH_BCS[v]) {
-    case bgk::BK_PERIODIC: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bgk::k_batch_tb<bgk::BK_PERIODIC>, 256, 0); break;
-    case bgk::BK_CHANNEL: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bgk::k_batch_tb<bgk::BK_CHANNEL>, 256, 0); break;
-    case bgk::BK_INFLOW: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bgk::k_batch_tb<bgk::BK_INFLOW>, 256, 0); break;
-    case bgk::BK_COUETTE: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bgk::k_batch_tb<bgk::BK_COUETTE>, 256, 0); break;
-    case bgk::BK_OPEN: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bgk::k_batch_tb<bgk::BK_OPEN>, 256, 0); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bgk::k_batch_tb<bgk::BK_DRIVEN>, 256, 0); break;
+    case bgk::BK_PERIODIC: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_PERIODIC>, 256, 0);
+    case bgk::BK_CHANNEL: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_CHANNEL>, 256, 0);
+    case bgk::BK_INFLOW: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_INFLOW>, 256, 0);
+    case bgk::BK_COUETTE: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_COUETTE>, 256, 0);
+    case bgk::BK_OPEN: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_OPEN>, 256, 0);
+    default: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_DRIVEN>, 256, 0);
     }
+}
+// blocks of the two-step batch kernel of storage kind `kind`, variant v,
+// that the device holds at once: its grid, since a persistent block that
+// starts late would run its whole share late
+int grid_tb(bgk_batch_t *b, int kind, int v)
+{
+    if (b->grid_tb[kind][v] > 0) return b->grid_tb[kind][v];
+    int per_cu = 0;
+    const int e = kind == bgk::MK_F16 ? occupancy_tb<_Float16>(v, &per_cu)
+                  : kind == bgk::MK_BF16 ? occupancy_tb<bgk::bf16_t>(v, &per_cu)
+                                         : occupancy_tb<float>(v, &per_cu);
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
-    b->grid_tb[v] = per_cu * b->cus;
-    return b->grid_tb[v];
+    b->grid_tb[kind][v] = per_cu * b->cus;
+    return b->grid_tb[kind][v];
 }
 }  // extern "C++"
 
@@ -1312,7 +1326,8 @@ int bgk_batch_create(int32_t device, double budget_bytes, int32_t max_members, b
         cus = 256;
     b->grid = cus * 8;
     b->cus = cus;
-    for (int v = 0; v < 6; v++) b->grid_tb[v] = 0;
+    for (int k = 0; k < 3; k++)
+        for (int v = 0; v < 6; v++) b->grid_tb[k][v] = 0;
     for (int g = 0; g < bgk::NGROUPS; g++) {
         b->gs[g] = nullptr;
         b->gev[g] = nullptr;
@@ -1446,25 +1461,39 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
     size_t nt = 0;
     std::vector<int> members_slot;
     std::vector<char> tbm;  // member advances two steps per launch in this chunk
-    size_t strip_rows = 0;
-    int tbv = -1;  // a variant of the two-step members
+    constexpr int G = bgk::NGROUPS;
+    size_t strip_rows[G] = {};  // per two-step group
     for (size_t q = 0; q < b->slots.size(); q++) {
         const auto &sl = b->slots[q];
         if (!sl.ctx || sl.remaining <= 0) continue;
         const bgk_ctx *c = sl.ctx;
         const bool tb = member_tb(c, sl.remaining, chunk) && !member_solo(c);
-        if (tb) {
-            strip_rows += (size_t)member_strips(c) * (size_t)c->m;
-            tbv = member_group(c, true) % 6;
-        }
+        if (tb) strip_rows[member_group(c, true)] += (size_t)member_strips(c) * (size_t)c->m;
         members_slot.push_back((int)q);
         tbm.push_back(tb);
         nm++;
     }
-    const int tbh = tbv >= 0 ? batch_tb_h(strip_rows, grid_tb(b, tbv)) : bgk::TB_BATCH_H;
+    // Two-step groups run one after another, each sized to fill the chip in
+    // whole rounds of strips of at least TB_HMIN rows.  A group too small
+    // for one such round would leave most of the chip idle while it runs:
+    // its members take the one-step kernels, whose groups overlap.
+    int tbh[G];
+    for (int g = 0; g < G; g++) {
+        tbh[g] = bgk::TB_BATCH_H;
+        if (!strip_rows[g]) continue;
+        const int slots = grid_tb(b, g / 6 - bgk::TWO_STEP, g % 6);
+        if (strip_rows[g] < (size_t)TB_HMIN * (size_t)slots) {
+            for (int mi = 0; mi < nm; mi++)
+                if (tbm[mi] && member_group(b->slots[members_slot[mi]].ctx, true) == g) tbm[mi] = 0;
+            strip_rows[g] = 0;
+            continue;
+        }
+        tbh[g] = batch_tb_h(strip_rows[g], slots);
+    }
+    auto height = [&](int mi) { return tbm[mi] ? tbh[member_group(b->slots[members_slot[mi]].ctx, true)] : 0; };
     for (int mi = 0; mi < nm; mi++) {
         const auto &sl = b->slots[members_slot[mi]];
-        nt += member_tasks(sl.ctx, sl.tw_log2, sl.th, tbm[mi], tbh);
+        nt += member_tasks(sl.ctx, sl.tw_log2, sl.th, tbm[mi], height(mi));
     }
     if (finished) *finished = 0;
     if (nm == 0) {
@@ -1474,11 +1503,10 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
     int r = ensure_tiles(b, nt);
     if (r) return r;
     // the task table holds the groups one after another
-    constexpr int G = bgk::NGROUPS;
     size_t cnt[G] = {}, pos[G];
     for (int mi = 0; mi < nm; mi++) {
         const auto &sl = b->slots[members_slot[mi]];
-        cnt[member_group(sl.ctx, tbm[mi])] += member_tasks(sl.ctx, sl.tw_log2, sl.th, tbm[mi], tbh);
+        cnt[member_group(sl.ctx, tbm[mi])] += member_tasks(sl.ctx, sl.tw_log2, sl.th, tbm[mi], height(mi));
     }
     for (int g = 0, acc = 0; g < G; g++) {
         pos[g] = (size_t)acc;
@@ -1511,8 +1539,8 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
         d.P = phys<float>(c);
         if (tbm[mi]) {
             d.tiles_x = member_strips(c);
-            d.th = tbh;
-            d.ntiles = (int)member_tb_tiles(c, tbh);
+            d.th = height(mi);
+            d.ntiles = (int)member_tb_tiles(c, d.th);
         } else {
             d.tiles_x = member_tiles_x(c, sl.tw_log2);
             d.th = sl.th;
@@ -1542,7 +1570,11 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
     }
     BHIPCHK(b, hipStreamWaitEvent(b->s, prev, 0));
     // each group's chain of `chunk` steps replays from its own graph on its
-    // own stream, after the tables are in place; the groups overlap
+    // own stream, after the tables are in place.  One-step groups overlap
+    // each other; two-step groups, each sized to fill the chip in whole
+    // rounds, run one after another (sharing the chip would leave a
+    // partial last round in each).
+    hipEvent_t prev_two = prev;
     for (int g = 0; g < G; g++) {
         if (!cnt[g]) continue;
         if (!b->gs[g]) {
@@ -1553,12 +1585,12 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
         // the chip to the other groups' chains
         // (two-step groups: always the full grid, whose surplus blocks exit
         // at once; their strip height already sizes the work to it)
-        const bool two = g / 6 == bgk::MK_F32TB;
-        const int full = two ? grid_tb(b, g % 6) : b->grid;
+        const bool two = g / 6 >= bgk::TWO_STEP;
+        const int full = two ? grid_tb(b, g / 6 - bgk::TWO_STEP, g % 6) : b->grid;
         const int nb = !two && cnt[g] <= SMALL_GROUP ? (int)SMALL_GROUP : full;
         const int nl = two ? chunk / 2 : chunk;  // launches of the chain
         const long long key = ((long long)chunk * G + g) * 2 + (nb == full);
-        BHIPCHK(b, hipStreamWaitEvent(b->gs[g], prev, 0));
+        BHIPCHK(b, hipStreamWaitEvent(b->gs[g], two ? prev_two : prev, 0));
         // a chain seen for the first time is launched directly; its graph
         // is captured when it comes again (capture and instantiation cost
         // more than the launches of one chunk)
@@ -1584,6 +1616,7 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
         }
         BHIPCHK(b, hipEventRecord(b->gev[g], b->gs[g]));
         BHIPCHK(b, hipStreamWaitEvent(b->s, b->gev[g], 0));
+        if (two) prev_two = b->gev[g];
     }
     BHIPCHK(b, hipEventRecord(b->ev1, b->s));
     BHIPCHK(b, hipEventSynchronize(b->ev1));

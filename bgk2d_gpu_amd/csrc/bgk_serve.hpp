@@ -34,10 +34,11 @@ enum MemberKind {
 
 // boundary variants the batch kernels serve, in group order
 constexpr int BATCH_BCS[6] = {BK_DRIVEN, BK_PERIODIC, BK_CHANNEL, BK_INFLOW, BK_COUETTE, BK_OPEN};
-// group = kind * 6 + index of the variant in BATCH_BCS; kind MK_F32TB holds
-// the fp32 members that advance two steps per launch in this chunk
-constexpr int MK_F32TB = 3;
-constexpr int NGROUPS = 4 * 6;
+// group = kind * 6 + index of the variant in BATCH_BCS; kinds TWO_STEP +
+// MK_* hold the members of that storage kind that advance two steps per
+// launch in this chunk
+constexpr int TWO_STEP = 3;
+constexpr int NGROUPS = 6 * 6;
 // largest strip height of the two-step batch kernel
 constexpr int TB_BATCH_H = 64;
 
@@ -123,7 +124,7 @@ __global__ __launch_bounds__(256) void k_batch(const Member *__restrict__ M, con
     }
 }
 
-// Two FUSED steps of every member of an MK_F32TB group per launch: a task
+// Two FUSED steps of every member of a two-step group per launch: a task
 // is one strip block of k_fused_tb (TB_TW columns by th rows), run by the
 // same device code, so the values are those of two one-step launches.
 // Launch t2 performs steps 2 t2 and 2 t2 + 1 of the chunk; a member in the
@@ -161,7 +162,7 @@ __device__ __forceinline__ Phys<float> cld_phys(const Phys<float> *p)
     return r;
 }
 
-template <int BC>
+template <class S, int BC>
 __global__ __launch_bounds__(256) void k_batch_tb(const Member *__restrict__ M, const int2 *__restrict__ tasks,
                                                   const int *__restrict__ meta, int group, int t2)
 {
@@ -197,7 +198,7 @@ __global__ __launch_bounds__(256) void k_batch_tb(const Member *__restrict__ M, 
         L.stepy = 0;
         const Phys<float> P = cld_phys(&d->P);
         const int tiles_x = cld(&d->tiles_x);
-        tb_block<float, float, BC, false>(L, P, cld(&d->th), tile % tiles_x, tile / tiles_x);
+        tb_block<S, float, BC, false>(L, P, cld(&d->th), tile % tiles_x, tile / tiles_x);
     }
 }
 

@@ -35,6 +35,7 @@ class Request:
     steps: int
     prec: str
     arrival: float = 0.0
+    tblock: int = 0  # bgk_params_t.tblock: 0 automatic, 1 one step per launch, 2 two
 
     def geometry(self):
         """(lx, ly, bc, svisc, u0, amplitude): velocity scale U = 0.1 for the
@@ -52,11 +53,11 @@ class Request:
         lx, ly = self.geometry()[:2]
         return lx * ly * self.steps
 
-    def params(self, device=0, form=1, rounding=1, tblock=0):
-        """bgk_params_t of the solve.  tblock=0 lets a large member (one the
-        batch advances with bgk_run's own kernels) use the two-step kernel
-        where that is faster, as a plain solve does; every path gives the
-        same bits."""
+    def params(self, device=0, form=1, rounding=1, tblock=None):
+        """bgk_params_t of the solve (tblock: the request's unless given).
+        The batch and bgk_run choose one or two steps per launch by the same
+        rule; every path gives the same bits."""
+        tblock = self.tblock if tblock is None else tblock
         lx, ly, bc, svisc, u0, _ = self.geometry()
         narrow = self.prec in ("fp16", "bf16")
         return BgkParams(lx=lx, ly=ly, bc=BC[bc], impl=IMPL["fused"], prec=PREC[self.prec],
@@ -305,7 +306,7 @@ def serve_sequential(reqs, check_every=1000, keep=(), device=0, clock=time.perf_
         narrow = r.prec in ("fp16", "bf16")
         s = Solver(lx, ly, bc=bc, impl="fused", prec=r.prec, svisc=svisc, u0=u0, amplitude=amp,
                    scale=1024.0 if narrow else 1.0, form=1 if narrow else 0,
-                   rounding=1 if narrow else 0, device=device)
+                   rounding=1 if narrow else 0, tblock=r.tblock, device=device)
         done, status = 0, "ok"
         while done < r.steps:
             n = min(check_every, r.steps - done)

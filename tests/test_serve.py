@@ -56,6 +56,27 @@ def test_batched_members_equal_solves_alone():
 
 
 @pytest.mark.gpu
+def test_two_step_batch_members_equal_solves_alone():
+    """groups large enough to fill the chip advance two steps per launch
+    (fp32 by default, 16-bit kinds with tblock = 2), their last partial
+    chunk one step per launch: each result is bit-identical to the same
+    request run alone"""
+    reqs = [Request(i, "cavity", 1024, 1000.0, 64 + 3 * i, "single", 0.0) for i in range(4)]
+    reqs += [Request(4 + i, "tgv", 1024, 500.0, 41 + 2 * i, "fp16", 0.0, tblock=2) for i in range(4)]
+    reqs += [Request(17, "poiseuille", 4096, 500.0, 30, "bf16", 0.0, tblock=2),
+             Request(18, "poiseuille", 4000, 500.0, 27, "bf16", 0.0, tblock=2),
+             Request(19, "cavity", 300, 1000.0, 50, "single", 0.0)]
+    keep = {r.rid for r in reqs}
+    bo, _ = serve.serve_batched(reqs, budget_bytes=60e9, chunk=10, check_every=100, keep=keep)
+    so, _ = serve.serve_sequential(reqs, check_every=100, keep=keep)
+    for r in reqs:
+        assert bo[r.rid].status == so[r.rid].status == "ok"
+        a, b = bo[r.rid].pops, so[r.rid].pops
+        assert np.array_equal(a[:, 1:-1, 1:-1], b[:, 1:-1, 1:-1]), r
+        np.testing.assert_array_equal(bo[r.rid].diag, so[r.rid].diag)
+
+
+@pytest.mark.gpu
 def test_admission_respects_budget_and_divergence_matches():
     """a budget that holds about two requests at a time: everything still
     completes, and an unstable request (tau near 0.5) diverges at the same
