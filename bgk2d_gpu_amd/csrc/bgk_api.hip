@@ -1531,8 +1531,6 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
         d.m = c->m;
         d.off = c->off;
         d.rs = c->pitch;
-        d.store = member_kind(c);
-        d.bc = c->p.bc;  // BGK_BC_* and BcKind share the numbering of the foldable variants
         d.tw_log2 = sl.tw_log2;
         d.parity0 = c->cur;
         d.remaining = (int)(sl.remaining < chunk ? sl.remaining : chunk);

@@ -56,9 +56,8 @@ constexpr int TASK_TILES = BGK_TASK_TILES;
 struct Member {
     long long set[2][8];  // the two population sets
     long long r[2];       // the two pop-19 planes
-    int l, m, off, store;
+    int l, m, off;
     long long rs;
-    int bc;                  // BcKind
     int tiles_x;             // tiles per row of tiles (strips for the two-step kernel)
     int ntiles;              // tiles of the member
     int tw_log2, th;         // tile shape: (1 << tw_log2) x th = 256 nodes (two-step: th = strip height)
