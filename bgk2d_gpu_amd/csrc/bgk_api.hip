@@ -1085,6 +1085,7 @@ struct bgk_batch {
         long long remaining, done;
         size_t bytes;
         int tw_log2, th;
+        bool held;  // skipped by bgk_batch_run
     };
     std::vector<Slot> slots;
     bgk::Member *d_members, *h_members;
@@ -1316,7 +1317,7 @@ int bgk_batch_create(int32_t device, double budget_bytes, int32_t max_members, b
     b->dev = device;
     b->budget = (size_t)budget_bytes;
     b->used = 0;
-    b->slots.assign((size_t)max_members, bgk_batch_t::Slot{nullptr, 0, 0, 0, 6, 4});
+    b->slots.assign((size_t)max_members, bgk_batch_t::Slot{nullptr, 0, 0, 0, 6, 4, false});
     b->tiles_cap = 0;
     b->d_tiles = nullptr;
     b->h_tiles = nullptr;
@@ -1414,7 +1415,7 @@ int bgk_batch_add(bgk_batch_t *b, const bgk_params_t *p, int32_t init_v, double 
         bgk_destroy(c);
         return r;
     }
-    b->slots[free_slot] = bgk_batch_t::Slot{c, (long long)nsteps, 0, bytes, tw_log2, th};
+    b->slots[free_slot] = bgk_batch_t::Slot{c, (long long)nsteps, 0, bytes, tw_log2, th, false};
     b->used += bytes;
     *slot = free_slot;
     return BGK_OK;
@@ -1440,7 +1441,15 @@ int bgk_batch_remove(bgk_batch_t *b, int32_t slot)
     (void)hipStreamSynchronize(b->s);
     bgk_destroy(b->slots[slot].ctx);
     b->used -= b->slots[slot].bytes;
-    b->slots[slot] = bgk_batch_t::Slot{nullptr, 0, 0, 0, 6, 4};
+    b->slots[slot] = bgk_batch_t::Slot{nullptr, 0, 0, 0, 6, 4, false};
+    return BGK_OK;
+}
+
+int bgk_batch_hold(bgk_batch_t *b, int32_t slot, int32_t held)
+{
+    if (!b || slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].ctx || held < 0 || held > 1)
+        return BGK_E_INVALID;
+    b->slots[slot].held = held != 0;
     return BGK_OK;
 }
 
@@ -1465,7 +1474,7 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
     size_t strip_rows[G] = {};  // per two-step group
     for (size_t q = 0; q < b->slots.size(); q++) {
         const auto &sl = b->slots[q];
-        if (!sl.ctx || sl.remaining <= 0) continue;
+        if (!sl.ctx || sl.remaining <= 0 || sl.held) continue;
         const bgk_ctx *c = sl.ctx;
         const bool tb = member_tb(c, sl.remaining, chunk) && !member_solo(c);
         if (tb) strip_rows[member_group(c, true)] += (size_t)member_strips(c) * (size_t)c->m;

@@ -108,7 +108,8 @@ class Batch:
         for name, args in (("bgk_batch_create", [I32, D, I32, P(V)]), ("bgk_batch_destroy", [V]),
                            ("bgk_batch_add", [V, P(BgkParams), I32, D, I64, I32, P(I32)]),
                            ("bgk_batch_ctx", [V, I32, P(V)]), ("bgk_batch_remaining", [V, I32, P(I64)]),
-                           ("bgk_batch_remove", [V, I32]), ("bgk_batch_run", [V, I32, P(I32)]),
+                           ("bgk_batch_remove", [V, I32]), ("bgk_batch_hold", [V, I32, I32]),
+                           ("bgk_batch_run", [V, I32, P(I32)]),
                            ("bgk_batch_used", [V, P(D)]), ("bgk_batch_last_ms", [V, P(D)]),
                            ("bgk_footprint", [P(BgkParams), P(D)])):
             getattr(L, name).argtypes = args
@@ -164,6 +165,11 @@ class Batch:
 
     def remove(self, slot):
         self.L.bgk_batch_remove(self.h, slot)
+
+    def hold(self, slot, held):
+        rc = self.L.bgk_batch_hold(self.h, slot, 1 if held else 0)
+        if rc != 0:
+            raise BgkError(rc, self.L.bgk_batch_last_error(self.h).decode())
 
     def last_ms(self):
         v = C.c_double(0)
@@ -227,8 +233,14 @@ def autotune(prec, side, steps=64, chunk=32, device=0, cache=True):
 
 
 def serve_batched(reqs, budget_bytes=200e9, chunk=100, check_every=1000, max_members=512,
-                  tiles=None, keep=(), device=0, clock=time.perf_counter):
+                  tiles=None, keep=(), device=0, clock=time.perf_counter, fill_nodes=None):
     """Continuous batching of `reqs` (arrival times relative to the start).
+    fill_nodes=None advances every admitted request each chunk (the most
+    aggregate throughput).  With fill_nodes, each chunk advances only the
+    admitted requests with the least remaining work (nodes x steps left)
+    whose lattices together hold at least fill_nodes nodes and holds the
+    others (shortest remaining work first: short requests are not slowed
+    by long ones sharing every step; a lower p50 for less throughput).
     Returns (outcomes by rid, stats)."""
     assert check_every % chunk == 0
     L = load()
@@ -253,17 +265,35 @@ def serve_batched(reqs, budget_bytes=200e9, chunk=100, check_every=1000, max_mem
                     continue
                 break
             out[r.rid].start = clock() - t0
-            active[slot] = [r, 0]
+            active[slot] = [r, 0, False]  # request, steps done, held
             qi += 1
         if not active:
             if qi < len(pending):
                 time.sleep(max(0.0, min(0.01, pending[qi].arrival - (clock() - t0))))
             continue
+        if fill_nodes:
+            def work(slot):
+                lx, ly = active[slot][0].geometry()[:2]
+                return lx * ly * b.remaining(slot), lx * ly
+            order = sorted(active, key=lambda sl: work(sl)[0])
+            nodes, run = 0, set()
+            for sl in order:
+                run.add(sl)
+                nodes += work(sl)[1]
+                if nodes >= fill_nodes:
+                    break
+            for sl in active:
+                held = sl not in run
+                if held != active[sl][2]:
+                    b.hold(sl, held)
+                    active[sl][2] = held
         b.run(chunk)
         busy += b.last_ms() / 1e3
         done_now = clock() - t0
         for slot in list(active):
-            r, done = active[slot]
+            if active[slot][2]:  # held this chunk
+                continue
+            r, done = active[slot][:2]
             rem = b.remaining(slot)
             done = r.steps - rem
             active[slot][1] = done

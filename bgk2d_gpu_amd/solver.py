@@ -28,7 +28,7 @@ EXPORTS = ("bgk_create", "bgk_destroy", "bgk_last_error", "bgk_version", "bgk_co
            "bgk_fields", "bgk_check",
            "bgk_download", "bgk_upload", "bgk_storage_bytes", "bgk_stream", "bgk_last_run_ms",
            "bgk_batch_create", "bgk_batch_destroy", "bgk_batch_last_error", "bgk_footprint",
-           "bgk_batch_add", "bgk_batch_ctx", "bgk_batch_remaining", "bgk_batch_remove",
+           "bgk_batch_add", "bgk_batch_ctx", "bgk_batch_remaining", "bgk_batch_remove", "bgk_batch_hold",
            "bgk_batch_used", "bgk_batch_run", "bgk_batch_last_ms")
 
 

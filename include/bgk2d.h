@@ -192,12 +192,14 @@ int bgk_last_run_ms(const bgk_ctx_t *ctx, double *ms);
 /* ------------------------------------------------------------------------
  * Batched solving (no counterpart in the reference, which runs one solve
  * per process): many independent FUSED solves of any size, boundary
- * (driven, periodic, channel, inflow, couette; no obstacle) and storage kind
- * (fp32, fp16, bf16) stay resident in HBM and advance together, one
- * persistent-kernel launch per step for all of them, chunks of steps
- * replayed from HIP graphs.  A member's results are bit-identical to the
- * same solve run alone with bgk_run.  Members are admitted while their
- * device footprint fits the byte budget given at creation.               */
+ * (driven, periodic, channel, inflow, couette, open; no obstacle) and
+ * storage kind (fp32, fp16, bf16) stay resident in HBM and advance
+ * together: per step one persistent-kernel launch per group of members of
+ * one storage kind and boundary variant (two steps per launch where
+ * bgk_run would use the two-step kernel), chunks of steps replayed from
+ * HIP graphs.  A member's results are bit-identical to the same solve run
+ * alone with bgk_run.  Members are admitted while their device footprint
+ * fits the byte budget given at creation.                                */
 typedef struct bgk_batch bgk_batch_t;
 
 int bgk_batch_create(int32_t device, double budget_bytes, int32_t max_members, bgk_batch_t **out);
@@ -214,6 +216,10 @@ int bgk_batch_add(bgk_batch_t *b, const bgk_params_t *p, int32_t init_v, double 
 int bgk_batch_ctx(bgk_batch_t *b, int32_t slot, bgk_ctx_t **ctx);
 int bgk_batch_remaining(bgk_batch_t *b, int32_t slot, int64_t *remaining);
 int bgk_batch_remove(bgk_batch_t *b, int32_t slot);
+/* held = 1: bgk_batch_run skips the member (its state and steps left stay
+ * as they are) until it is released with held = 0; for a request loop's
+ * scheduling policy                                                      */
+int bgk_batch_hold(bgk_batch_t *b, int32_t slot, int32_t held);
 int bgk_batch_used(const bgk_batch_t *b, double *bytes);
 /* advance every member with steps left by min(chunk, remaining) steps   */
 int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished);

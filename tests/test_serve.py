@@ -77,6 +77,26 @@ def test_two_step_batch_members_equal_solves_alone():
 
 
 @pytest.mark.gpu
+def test_shortest_first_holds_members_and_keeps_results():
+    """with a fill threshold the loop holds the requests with the most work
+    left: the short ones finish first, and every result is still
+    bit-identical to the request run alone"""
+    reqs = [Request(0, "cavity", 512, 1000.0, 300, "single", 0.0),
+            Request(1, "tgv", 96, 300.0, 120, "fp16", 0.0),
+            Request(2, "poiseuille", 256, 200.0, 90, "single", 0.0),
+            Request(3, "cavity", 128, 100.0, 60, "bf16", 0.0)]
+    keep = {r.rid for r in reqs}
+    bo, _ = serve.serve_batched(reqs, budget_bytes=60e9, chunk=10, check_every=100, keep=keep,
+                                fill_nodes=20000)
+    so, _ = serve.serve_sequential(reqs, check_every=100, keep=keep)
+    for r in reqs:
+        assert bo[r.rid].status == so[r.rid].status == "ok"
+        assert np.array_equal(bo[r.rid].pops[:, 1:-1, 1:-1], so[r.rid].pops[:, 1:-1, 1:-1]), r
+    # the 512^2 cavity (most work) is held until the others are done
+    assert bo[0].done >= max(bo[r.rid].done for r in reqs[1:])
+
+
+@pytest.mark.gpu
 def test_admission_respects_budget_and_divergence_matches():
     """a budget that holds about two requests at a time: everything still
     completes, and an unstable request (tau near 0.5) diverges at the same
