@@ -27,6 +27,8 @@ struct bgk_ctx {
     size_t plane_elems;  // elements per plane, incl. alignment padding
     int dev;
     hipStream_t s;
+    hipStream_t s2;        // the obstacle's strip blocks of the two-step kernel
+    hipEvent_t evf, evj;   // fork to s2 / join back to s
     void *mem;
     void *plane[18];  // [0..7] set 0, [8..15] set 1, [16], [17] pop 19
     int cur;          // which set is `a`
@@ -308,6 +310,19 @@ bool use_tb(const bgk_ctx *c)
     return c->p.tblock == 0 && c->esize >= 4 && c->tb_fill;
 }
 
+// Strip blocks of height H that must apply the obstacle rule: those whose
+// updated nodes (their strip, its rim columns, one row above and below)
+// reach the obstacle's bounding box grown by one node, where fix_pulls
+// applies it -- with one more block on each side as a margin.
+// (first strip, last strip, first row, last row) of blocks; x > y: none.
+int4 tb_box(const bgk_ctx *c, int H)
+{
+    if (!(c->p.obstacle && c->nobs > 0)) return make_int4(0, -1, 0, -1);
+    const int strips = (c->l + TB_TW - 1) / TB_TW, rows = (c->m + H - 1) / H;
+    return make_int4(max(0, (c->imin - 3) / TB_TW - 1), min(strips - 1, (c->imax + 1) / TB_TW + 1),
+                     max(0, (c->jmin - 3) / H - 1), min(rows - 1, (c->jmax + 1) / H + 1));
+}
+
 // Strip height of k_fused_tb: as many strips of rows as make the grid one
 // round of the blocks the device holds at once, so no CU idles in a
 // partial last round; at least TB_HMIN rows, since each strip also
@@ -316,7 +331,10 @@ template <class S, class C, int BC, bool OBS>
 int size_tb(bgk_ctx *c)
 {
     int per_cu = 0, cus = 0;
-    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused_tb<S, C, BC, OBS>, 256, 0));
+    // with an obstacle the grid is still that of the obstacle-free kernel:
+    // the obstacle's few strip blocks run beside it (launch_tb_t); leaving
+    // them room in the round measured slower (91 k against 94 k MLUPS)
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused_tb<S, C, BC, false>, 256, 0));
     HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->dev));
     if (per_cu < 1) per_cu = 1;
     if (cus < 1) cus = 1;
@@ -352,10 +370,24 @@ int launch_tb_t(bgk_ctx *c)
     }
     Lat L = lat(c);
     Phys<C> P = phys<C>(c);
-    const int strips = (c->l + TB_TW - 1) / TB_TW, rows = (c->m + c->tb_h - 1) / c->tb_h;
+    const int H = c->tb_h;
+    const int strips = (c->l + TB_TW - 1) / TB_TW, rows = (c->m + H - 1) / H;
     const int rows8 = (rows + NXCD - 1) / NXCD * NXCD;
-    hipLaunchKernelGGL((k_fused_tb<S, C, BC, OBS>), dim3(strips * rows8), dim3(256), 0, c->s, L, P, c->tb_h, strips,
-                       rows);
+    const int4 box = OBS ? tb_box(c, H) : make_int4(0, -1, 0, -1);  // strip blocks with the obstacle rule
+    if (OBS && box.y >= box.x) {
+        // the obstacle's blocks on a second stream, beside the others
+        // the rows of the skipped blocks, in blocks of a quarter of H
+        const int nbx = box.y - box.x + 1, r0 = box.z * H + 1, r1 = min((box.w + 1) * H, c->m);
+        const int hb = max(TB_HMIN, H / 4), nsub = (r1 - r0 + hb) / hb;
+        HIPCHK(c, hipEventRecord(c->evf, c->s));
+        HIPCHK(c, hipStreamWaitEvent(c->s2, c->evf, 0));
+        hipLaunchKernelGGL((k_fused_tb_box<S, C, BC>), dim3(nbx * nsub), dim3(256), 0, c->s2, L, P, box.x, nbx, r0, r1,
+                           hb);
+        HIPCHK(c, hipEventRecord(c->evj, c->s2));
+    }
+    hipLaunchKernelGGL((k_fused_tb<S, C, BC, false>), dim3(strips * rows8), dim3(256), 0, c->s, L, P, H, strips, rows,
+                       box);
+    if (OBS && box.y >= box.x) HIPCHK(c, hipStreamWaitEvent(c->s, c->evj, 0));
     c->cur = 1 - c->cur;
     c->rcur = 1 - c->rcur;
     HIPCHK(c, hipGetLastError());
@@ -690,12 +722,20 @@ int bgk_create(const bgk_params_t *p, bgk_ctx_t **out)
     }
     hipEventCreate(&c->ev0);
     hipEventCreate(&c->ev1);
+    if (hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess) {
+        hipStreamDestroy(c->s);
+        delete c;
+        return BGK_E_HIP;
+    }
+    hipEventCreateWithFlags(&c->evf, hipEventDisableTiming);
+    hipEventCreateWithFlags(&c->evj, hipEventDisableTiming);
     size_t bytes = c->plane_elems * (size_t)c->esize * 18;
     size_t obs_bytes = ((size_t)c->l * c->m + 255) / 256 * 256;
     size_t rows_bytes = sizeof(double) * ROW_VALS * (size_t)c->m;
     size_t red_bytes = sizeof(double) * 3 * (size_t)(c->l > c->m ? c->l : c->m) + 64;
     if (hipMalloc(&c->mem, bytes) != hipSuccess) {
         hipStreamDestroy(c->s);
+        hipStreamDestroy(c->s2);
         delete c;
         return BGK_E_NOMEM;
     }
@@ -703,6 +743,7 @@ int bgk_create(const bgk_params_t *p, bgk_ctx_t **out)
         hipMalloc(&c->d_red, red_bytes) != hipSuccess || hipHostMalloc(&c->h_red, red_bytes) != hipSuccess) {
         hipFree(c->mem);
         hipStreamDestroy(c->s);
+        hipStreamDestroy(c->s2);
         delete c;
         return BGK_E_NOMEM;
     }
@@ -749,7 +790,10 @@ int bgk_destroy(bgk_ctx_t *c)
     if (c->d_fields) hipFree(c->d_fields);
     hipEventDestroy(c->ev0);
     hipEventDestroy(c->ev1);
+    hipEventDestroy(c->evf);
+    hipEventDestroy(c->evj);
     hipStreamDestroy(c->s);
+    hipStreamDestroy(c->s2);
     delete c;
     return BGK_OK;
 }

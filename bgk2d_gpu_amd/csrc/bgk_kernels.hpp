@@ -737,9 +737,9 @@ __device__ __forceinline__ void tb_load(const Lat &L, int i, int j, R xr[9])
     xr[8] = *(const R *)(pull_base<S>(L, L.r, 8) + bo);
 }
 
-// the strip block (bx, by): columns bx*TB_TW+1.., rows by*H+1..
+// the strip block of strip bx (columns bx*TB_TW+1..) and rows j0..jend
 template <class S, class C, int BC, bool OBS>
-__device__ __forceinline__ void tb_block(const Lat &L, const Phys<C> &P, int H, int bx, int by)
+__device__ __forceinline__ void tb_block(const Lat &L, const Phys<C> &P, int bx, int j0, int jend)
 {
     using R = typename CodecT<S, C>::raw;
     constexpr int TW = TB_TW, E = TW + 2;
@@ -748,8 +748,8 @@ __device__ __forceinline__ void tb_block(const Lat &L, const Phys<C> &P, int H, 
     constexpr int RB = TB_RB;
     __shared__ R rim[2][RB][3];
     const int l = L.l, m = L.m;
-    const int i0 = bx * TW + 1, j0 = by * H + 1;
-    const int ncols = min(TW, l - i0 + 1), jend = min(j0 + H - 1, m);
+    const int i0 = bx * TW + 1;
+    const int ncols = min(TW, l - i0 + 1);
     const int tid = threadIdx.x;
     // node row / column that a ring position holds, -1 for a halo of a
     // wall-bounded direction
@@ -852,12 +852,28 @@ __device__ __forceinline__ bool tb_xcd_block(int b, int strips, int rows, int &b
     return by < rows;
 }
 
+// box = (first, last strip; first, last row of strip blocks): blocks left
+// to k_fused_tb_box (an empty range, x > y, skips none)
 template <class S, class C, int BC, bool OBS>
-__global__ __launch_bounds__(256) void k_fused_tb(Lat L, Phys<C> P, int H, int strips, int rows)
+__global__ __launch_bounds__(256) void k_fused_tb(Lat L, Phys<C> P, int H, int strips, int rows, int4 box)
 {
     int bx, by;
     if (!tb_xcd_block((int)blockIdx.x, strips, rows, bx, by)) return;
-    tb_block<S, C, BC, OBS>(L, P, H, bx, by);
+    if (bx >= box.x && bx <= box.y && by >= box.z && by <= box.w) return;
+    tb_block<S, C, BC, OBS>(L, P, bx, by * H + 1, min(by * H + H, L.m));
+}
+
+// The strip blocks that reach the obstacle, with the obstacle rule: strips
+// bx0.. (nbx of them) over rows r0..r1, cut into blocks of hb rows.
+// Launched beside k_fused_tb on a second stream: the rule's code raises
+// the register count (to ~100) and the time per node, so only these blocks
+// carry it, shorter than the others so that they end with them, and the
+// grid of the others keeps its occupancy.
+template <class S, class C, int BC>
+__global__ __launch_bounds__(256) void k_fused_tb_box(Lat L, Phys<C> P, int bx0, int nbx, int r0, int r1, int hb)
+{
+    const int j0 = r0 + ((int)blockIdx.x / nbx) * hb;
+    tb_block<S, C, BC, true>(L, P, bx0 + (int)blockIdx.x % nbx, j0, min(j0 + hb - 1, r1));
 }
 
 // ------------------------------------------------------------ diagnostics

@@ -197,7 +197,8 @@ __global__ __launch_bounds__(256) void k_batch_tb(const Member *__restrict__ M, 
         L.stepy = 0;
         const Phys<float> P = cld_phys(&d->P);
         const int tiles_x = cld(&d->tiles_x);
-        tb_block<S, float, BC, false>(L, P, cld(&d->th), tile % tiles_x, tile / tiles_x);
+        const int th = cld(&d->th), j0 = (tile / tiles_x) * th + 1;
+        tb_block<S, float, BC, false>(L, P, tile % tiles_x, j0, min(j0 + th - 1, L.m));
     }
 }
 
