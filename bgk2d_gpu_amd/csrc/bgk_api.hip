@@ -14,6 +14,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -1138,9 +1139,16 @@ struct bgk_batch {
     size_t tiles_cap;
     std::map<long long, hipGraphExec_t> graphs;  // key: (chunk, group, full grid)
     std::map<long long, int> uses;
-    hipStream_t gs[bgk::NGROUPS];          // one stream per group, made on first use
+    // The groups' chains run on NQ streams made one after another at
+    // creation; each chunk assigns the groups to them by work.  With the
+    // batch's own stream they take the four hardware queues a process gets
+    // by default, one each: on the mixed stream 3 gave 68.6 k MLUPS run
+    // after run, 2 gave 63 k, 1 gave 49 k, and 4 (two streams then share a
+    // queue) 61-72 k depending on which.
+    static constexpr int NQ = 3;
+    hipStream_t qs[NQ];
+    hipEvent_t qev[NQ];
     int cus, grid_tb[3][6];
-    hipEvent_t gev[bgk::NGROUPS];
     int grid;
     hipEvent_t ev0, ev1;
     double last_ms;
@@ -1240,13 +1248,12 @@ int member_group(const bgk_ctx *c, bool tb)
 // one step of a group's members
 extern "C++" {
 template <class S>
-void launch_batch_s(bgk_batch_t *b, int g, int nb, int t)
+void launch_batch_s(bgk_batch_t *b, int g, int nb, int t, hipStream_t s)
 {
     const dim3 grid(nb), blk(256);
     const bgk::Member *dm = b->d_members;
     const int2 *dt = b->d_tiles;
     const int *meta = b->d_ntiles;
-    hipStream_t s = b->gs[g];
     switch (bgk::BATCH_BCS[g % 6]) {
     case bgk::BK_PERIODIC: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_PERIODIC>), grid, blk, 0, s, dm, dt, meta, g, t); break;
     case bgk::BK_CHANNEL: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_CHANNEL>), grid, blk, 0, s, dm, dt, meta, g, t); break;
@@ -1257,13 +1264,12 @@ void launch_batch_s(bgk_batch_t *b, int g, int nb, int t)
     }
 }
 template <class S>
-void launch_batch_tb(bgk_batch_t *b, int g, int nb, int t2)
+void launch_batch_tb(bgk_batch_t *b, int g, int nb, int t2, hipStream_t s)
 {
     const dim3 grid(nb), blk(256);
     const bgk::Member *dm = b->d_members;
     const int2 *dt = b->d_tiles;
     const int *meta = b->d_ntiles;
-    hipStream_t s = b->gs[g];
     switch (bgk::BATCH_BCS[g % 6]) {
     case bgk::BK_PERIODIC: hipLaunchKernelGGL((bgk::k_batch_tb<S, bgk::BK_PERIODIC>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
     case bgk::BK_CHANNEL: hipLaunchKernelGGL((bgk::k_batch_tb<S, bgk::BK_CHANNEL>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
@@ -1274,15 +1280,15 @@ void launch_batch_tb(bgk_batch_t *b, int g, int nb, int t2)
     }
 }
 // launch t of a group's chain (a two-step group's chain has chunk / 2)
-void launch_batch(bgk_batch_t *b, int g, int nb, int t)
+void launch_batch(bgk_batch_t *b, int g, int nb, int t, hipStream_t s)
 {
     switch (g / 6) {
-    case bgk::MK_F16: launch_batch_s<_Float16>(b, g, nb, t); break;
-    case bgk::MK_BF16: launch_batch_s<bgk::bf16_t>(b, g, nb, t); break;
-    case bgk::TWO_STEP + bgk::MK_F32: launch_batch_tb<float>(b, g, nb, t); break;
-    case bgk::TWO_STEP + bgk::MK_F16: launch_batch_tb<_Float16>(b, g, nb, t); break;
-    case bgk::TWO_STEP + bgk::MK_BF16: launch_batch_tb<bgk::bf16_t>(b, g, nb, t); break;
-    default: launch_batch_s<float>(b, g, nb, t); break;
+    case bgk::MK_F16: launch_batch_s<_Float16>(b, g, nb, t, s); break;
+    case bgk::MK_BF16: launch_batch_s<bgk::bf16_t>(b, g, nb, t, s); break;
+    case bgk::TWO_STEP + bgk::MK_F32: launch_batch_tb<float>(b, g, nb, t, s); break;
+    case bgk::TWO_STEP + bgk::MK_F16: launch_batch_tb<_Float16>(b, g, nb, t, s); break;
+    case bgk::TWO_STEP + bgk::MK_BF16: launch_batch_tb<bgk::bf16_t>(b, g, nb, t, s); break;
+    default: launch_batch_s<float>(b, g, nb, t, s); break;
     }
 }
 // blocks of the two-step batch kernel the device holds at once: its grid,
@@ -1373,13 +1379,18 @@ int bgk_batch_create(int32_t device, double budget_bytes, int32_t max_members, b
     b->cus = cus;
     for (int k = 0; k < 3; k++)
         for (int v = 0; v < 6; v++) b->grid_tb[k][v] = 0;
-    for (int g = 0; g < bgk::NGROUPS; g++) {
-        b->gs[g] = nullptr;
-        b->gev[g] = nullptr;
-    }
     if (hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess) {
         delete b;
         return BGK_E_HIP;
+    }
+    for (int q = 0; q < bgk_batch_t::NQ; q++) {
+        if (hipStreamCreateWithFlags(&b->qs[q], hipStreamNonBlocking) != hipSuccess) {
+            for (int k = 0; k < q; k++) (void)hipStreamDestroy(b->qs[k]);
+            (void)hipStreamDestroy(b->s);
+            delete b;
+            return BGK_E_HIP;
+        }
+        (void)hipEventCreateWithFlags(&b->qev[q], hipEventDisableTiming);
     }
     if (hipMalloc(&b->d_members, sizeof(bgk::Member) * max_members) != hipSuccess ||
         hipHostMalloc(&b->h_members, sizeof(bgk::Member) * max_members) != hipSuccess ||
@@ -1411,10 +1422,10 @@ int bgk_batch_destroy(bgk_batch_t *b)
     (void)hipHostFree(b->h_ntiles);
     (void)hipEventDestroy(b->ev0);
     (void)hipEventDestroy(b->ev1);
-    for (int g = 0; g < bgk::NGROUPS; g++) {
-        if (b->gs[g]) (void)hipStreamSynchronize(b->gs[g]);
-        if (b->gev[g]) (void)hipEventDestroy(b->gev[g]);
-        if (b->gs[g]) (void)hipStreamDestroy(b->gs[g]);
+    for (int q = 0; q < bgk_batch_t::NQ; q++) {
+        (void)hipStreamSynchronize(b->qs[q]);
+        (void)hipEventDestroy(b->qev[q]);
+        (void)hipStreamDestroy(b->qs[q]);
     }
     (void)hipStreamDestroy(b->s);
     delete b;
@@ -1557,6 +1568,7 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
     if (r) return r;
     // the task table holds the groups one after another
     size_t cnt[G] = {}, pos[G];
+    double gnodes[G] = {};  // nodes a group updates per step
     for (int mi = 0; mi < nm; mi++) {
         const auto &sl = b->slots[members_slot[mi]];
         cnt[member_group(sl.ctx, tbm[mi])] += member_tasks(sl.ctx, sl.tw_log2, sl.th, tbm[mi], height(mi));
@@ -1598,6 +1610,7 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
             d.ntiles = (int)member_tiles(c, sl.tw_log2, sl.th);
         }
         if (member_solo(c)) continue;
+        gnodes[member_group(c, tbm[mi])] += (double)c->l * (double)c->m;
         size_t &t = pos[member_group(c, tbm[mi])];
         const int per = tbm[mi] ? 1 : bgk::TASK_TILES;
         for (int q = 0; q < d.ntiles; q += per) b->h_tiles[t++] = make_int2(mi, q);
@@ -1620,17 +1633,38 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
         prev = c->ev1;
     }
     BHIPCHK(b, hipStreamWaitEvent(b->s, prev, 0));
-    // each group's chain of `chunk` steps replays from its own graph on its
-    // own stream, after the tables are in place.  One-step groups overlap
-    // each other; two-step groups, each sized to fill the chip in whole
-    // rounds, run one after another (sharing the chip would leave a
-    // partial last round in each).
-    hipEvent_t prev_two = prev;
+    // Each group's chain of `chunk` steps replays from its own graph, after
+    // the tables are in place.  Two-step groups, each sized to fill the chip
+    // in whole rounds, run one after another on the first stream (sharing
+    // the chip would leave a partial last round in each); one-step groups
+    // go to the stream with the least work so far, largest first, and
+    // overlap with the other streams.
+    double load[bgk_batch_t::NQ] = {};
+    int order[G], ng = 0, where[G];
     for (int g = 0; g < G; g++) {
         if (!cnt[g]) continue;
-        if (!b->gs[g]) {
-            BHIPCHK(b, hipStreamCreateWithFlags(&b->gs[g], hipStreamNonBlocking));
-            BHIPCHK(b, hipEventCreateWithFlags(&b->gev[g], hipEventDisableTiming));
+        if (g / 6 >= bgk::TWO_STEP) {
+            where[g] = 0;
+            load[0] += 0.5 * gnodes[g];  // two steps per pass: about half the time per update
+        } else {
+            order[ng++] = g;
+        }
+    }
+    std::sort(order, order + ng, [&](int x, int y) { return gnodes[x] > gnodes[y]; });
+    for (int k = 0; k < ng; k++) {
+        int q = 0;
+        for (int u = 1; u < bgk_batch_t::NQ; u++)
+            if (load[u] < load[q]) q = u;
+        where[order[k]] = q;
+        load[q] += gnodes[order[k]];
+    }
+    bool used[bgk_batch_t::NQ] = {};
+    for (int g = 0; g < G; g++) {
+        if (!cnt[g]) continue;
+        hipStream_t qs = b->qs[where[g]];
+        if (!used[where[g]]) {
+            BHIPCHK(b, hipStreamWaitEvent(qs, prev, 0));
+            used[where[g]] = true;
         }
         // a group with few tasks launches fewer blocks, leaving the rest of
         // the chip to the other groups' chains
@@ -1641,21 +1675,20 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
         const int nb = !two && cnt[g] <= SMALL_GROUP ? (int)SMALL_GROUP : full;
         const int nl = two ? chunk / 2 : chunk;  // launches of the chain
         const long long key = ((long long)chunk * G + g) * 2 + (nb == full);
-        BHIPCHK(b, hipStreamWaitEvent(b->gs[g], two ? prev_two : prev, 0));
         // a chain seen for the first time is launched directly; its graph
         // is captured when it comes again (capture and instantiation cost
         // more than the launches of one chunk)
         if (++b->uses[key] < 2) {
-            for (int tt = 0; tt < nl; tt++) launch_batch(b, g, nb, tt);
+            for (int tt = 0; tt < nl; tt++) launch_batch(b, g, nb, tt, qs);
             BHIPCHK(b, hipGetLastError());
         } else {
             hipGraphExec_t ge;
             auto it = b->graphs.find(key);
             if (it == b->graphs.end()) {
-                BHIPCHK(b, hipStreamBeginCapture(b->gs[g], hipStreamCaptureModeThreadLocal));
-                for (int tt = 0; tt < nl; tt++) launch_batch(b, g, nb, tt);
+                BHIPCHK(b, hipStreamBeginCapture(qs, hipStreamCaptureModeThreadLocal));
+                for (int tt = 0; tt < nl; tt++) launch_batch(b, g, nb, tt, qs);
                 hipGraph_t gr = nullptr;
-                BHIPCHK(b, hipStreamEndCapture(b->gs[g], &gr));
+                BHIPCHK(b, hipStreamEndCapture(qs, &gr));
                 hipError_t e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(gr);
                 if (e != hipSuccess) return bfail(b, BGK_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
@@ -1663,11 +1696,13 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
             } else {
                 ge = it->second;
             }
-            BHIPCHK(b, hipGraphLaunch(ge, b->gs[g]));
+            BHIPCHK(b, hipGraphLaunch(ge, qs));
         }
-        BHIPCHK(b, hipEventRecord(b->gev[g], b->gs[g]));
-        BHIPCHK(b, hipStreamWaitEvent(b->s, b->gev[g], 0));
-        if (two) prev_two = b->gev[g];
+    }
+    for (int q = 0; q < bgk_batch_t::NQ; q++) {
+        if (!used[q]) continue;
+        BHIPCHK(b, hipEventRecord(b->qev[q], b->qs[q]));
+        BHIPCHK(b, hipStreamWaitEvent(b->s, b->qev[q], 0));
     }
     BHIPCHK(b, hipEventRecord(b->ev1, b->s));
     BHIPCHK(b, hipEventSynchronize(b->ev1));
