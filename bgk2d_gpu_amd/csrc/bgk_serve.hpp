@@ -43,9 +43,11 @@ constexpr int NGROUPS = 6 * 6;
 constexpr int TB_BATCH_H = 64;
 
 // tiles per task: enough to hide the member lookup, few enough that a
-// 128^2 member still spreads over several blocks
+// 128^2 member still spreads over several blocks (measured: 8 as fast as 4
+// for fp32 and 3-20 % slower for fp16, 16 slower for both; 1, 2, 32, 64
+// and strided tasks slower than 8)
 #ifndef BGK_TASK_TILES
-#define BGK_TASK_TILES 8
+#define BGK_TASK_TILES 4
 #endif
 constexpr int TASK_TILES = BGK_TASK_TILES;
 
