@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_layout_matches_header():
-    # bgk_params_t: 14 int32 then 3 doubles, no padding
+    # bgk_params_t: 16 int32 then 4 doubles, no padding
     assert ctypes.sizeof(pkg.solver.BgkParams) == 16 * 4 + 4 * 8
 
 
@@ -37,3 +37,7 @@ def test_null_arguments_rejected_without_gpu():
     assert L.bgk_create(None, None) == -1
     assert L.bgk_destroy(None) == -1
     assert L.bgk_constants(None, None) == -1
+    for f, args in (("bgk_batch_hold", (None, 0, 1)), ("bgk_batch_remove", (None, 0)),
+                    ("bgk_batch_run", (None, 10, None)), ("bgk_batch_destroy", (None,)),
+                    ("bgk_draglift", (None, 1, 2, 1, 2, None)), ("bgk_fields", (None, None, None, None))):
+        assert getattr(L, f)(*args) == -1, f
