@@ -62,13 +62,12 @@ __host__ __device__ inline uint32_t salt_of(int i, int j, int k)
 {
     return mul24((uint32_t)i, 0x9E3779u) + mul24((uint32_t)j, 0x85EBCAu) + (uint32_t)k * 0x9E3779B9u;
 }
-// per-store offset: one 24-bit multiply (full rate on CDNA) of the low bits
-// of value^salt, which change from step to step, then an xorshift; bits
-// 8.. of the result are the offset
-__host__ __device__ inline uint32_t sr_hash(uint32_t bits, uint32_t salt)
+// per-store random word: one 24-bit multiply (full rate on CDNA) of the
+// low bits of value^salt, which change from step to step; the conversion
+// takes its top bits, the best-mixed bits of the product
+__host__ __device__ inline uint32_t sr_word(uint32_t bits, uint32_t salt)
 {
-    uint32_t h = mul24(bits ^ salt, 0xB5297Au);
-    return h ^ (h >> 15);
+    return mul24(bits ^ salt, 0xB5297Au);
 }
 
 typedef __fp16 sr_half2 __attribute__((ext_vector_type(2)));
@@ -104,8 +103,7 @@ struct Codec<_Float16, float> {
     {
         const float x = v * sc;
         if (sr) {
-            // offset bits 8..20 of the hash as the top 13 bits of the operand
-            const sr_half2 h = __builtin_amdgcn_cvt_sr_f16_f32(sr_half2{}, x, sr_hash(__float_as_uint(x), salt) << 11, false);
+            const sr_half2 h = __builtin_amdgcn_cvt_sr_f16_f32(sr_half2{}, x, sr_word(__float_as_uint(x), salt), false);
             return (_Float16)h[0];
         }
         return (_Float16)x;
@@ -124,8 +122,7 @@ struct Codec<bf16_t, float> {
         uint32_t x = __float_as_uint(f);
         bf16_t r;
         if (sr) {
-            // offset bits 8..23 of the hash as the top 16 bits of the operand
-            const sr_short2 h = __builtin_amdgcn_cvt_sr_bf16_f32(sr_short2{}, f, sr_hash(x, salt) << 8, false);
+            const sr_short2 h = __builtin_amdgcn_cvt_sr_bf16_f32(sr_short2{}, f, sr_word(x, salt), false);
             r.bits = (uint16_t)h[0];
         } else if ((x & 0x7fffffffu) > 0x7f800000u) {
             r.bits = (uint16_t)((x >> 16) | 0x40u);
