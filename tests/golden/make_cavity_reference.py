@@ -1,0 +1,85 @@
+"""Cavity fixtures from the reference's own code, built SINGLE and DOUBLE
+(oracle/_ref/ref_{SINGLE,DOUBLE}_{ORIG,FUSED}_LDC):
+
+  c1: 256^2, ORIGINAL stream + collide, Re = 100 (svisc = 0.256, u0 = 0.1),
+      10,000 steps (seconds);
+  c2: 4096^2, FUSED, Re = 1e4 (svisc = 0.04096, u0 = 0.1), 20,000 steps
+      (about an hour of one core for both builds).
+
+Stores only digests and derived numbers (<case>_reference.json): the
+SHA-256 of each build's interior populations (raw storage bits,
+population-major, rows j = 1..m, columns i = 1..l), the velocity/density
+fields' Linf and relative L2 differences between the two builds, and the
+total mass of each.  Nothing of the reference's source.
+
+    make -C oracle ref && python tests/golden/make_cavity_reference.py [c1|c2]
+"""
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import pyoracle  # noqa: E402
+
+CASES = {"c1": ("ORIG", 256, 256, 0.256, 0.1, 10000), "c2": ("FUSED", 4096, 4096, 0.04096, 0.1, 20000)}
+# D2Q9 in the order 01 03 05 08 10 12 14 17 19
+CX = np.array([1, 1, 1, 0, -1, -1, -1, 0, 0])
+CY = np.array([-1, 1, 0, 1, -1, 1, 0, -1, 0])
+
+
+def digest(pops):
+    """SHA-256 of the interior populations, raw bits"""
+    return hashlib.sha256(np.ascontiguousarray(pops[:, 1:-1, 1:-1]).tobytes()).hexdigest()
+
+
+def fields(pops):
+    """rho, u, v in FP64 from the shifted populations (f - w)"""
+    f = pops[:, 1:-1, 1:-1].astype(np.float64)
+    rho = f.sum(0) + 1.0
+    return rho, (CX[:, None, None] * f).sum(0) / rho, (CY[:, None, None] * f).sum(0) / rho
+
+
+def compare(a, b):
+    """Linf and relative L2 of (u, v) and Linf of rho between two runs"""
+    ra, ua, va = fields(a)
+    rb, ub, vb = fields(b)
+    du = np.concatenate([(ua - ub).ravel(), (va - vb).ravel()])
+    ref = np.concatenate([ub.ravel(), vb.ravel()])
+    return dict(uv_linf=float(np.abs(du).max()), uv_rel_l2=float(np.linalg.norm(du) / np.linalg.norm(ref)),
+                rho_linf=float(np.abs(ra - rb).max()))
+
+
+def run(prec, impl, l, m, svisc, u0, steps):
+    exe = os.path.join(ROOT, "oracle", "_ref", f"ref_{prec}_{impl}_LDC")
+    with tempfile.TemporaryDirectory() as td:
+        dump = os.path.join(td, "dump.bin")
+        subprocess.run([exe, str(l), str(m), repr(svisc), repr(u0), str(steps), "16", dump], cwd=td,
+                       check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        return pyoracle.read_ref_dump(dump)["pops"]
+
+
+def main():
+    name = sys.argv[1] if len(sys.argv) > 1 else "c1"
+    impl, l, m, svisc, u0, steps = CASES[name]
+    s = run("SINGLE", impl, l, m, svisc, u0, steps)
+    d = run("DOUBLE", impl, l, m, svisc, u0, steps)
+    out = dict(case=f"lid-driven cavity {l}x{m}, {'ORIGINAL' if impl == 'ORIG' else impl}, "
+                    f"svisc={svisc}, u0={u0}, {steps} steps",
+               sha256_single=digest(s), sha256_double=digest(d),
+               single_vs_double=compare(s, d),
+               mass_single=float(fields(s)[0].astype(np.float64).sum()),
+               mass_double=float(fields(d)[0].sum()))
+    with open(os.path.join(HERE, f"{name}_reference.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out, indent=1))
+
+
+if __name__ == "__main__":
+    main()

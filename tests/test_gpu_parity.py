@@ -276,3 +276,42 @@ def test_cavity_re100_256_against_reference_profile_and_ghia():
     assert np.max(np.abs(np.interp(gy[sel, 0], y, um) - gy[sel, 1])) < 0.015
     d = s.diagno()
     assert abs(d[2] - 1.0) < 1e-4
+
+
+@pytest.mark.gpu
+def test_c1_cavity_256_original_matches_reference_builds():
+    """C1: lid-driven cavity 256^2, ORIGINAL stream + collide, Re = 100
+    (svisc = 0.256, u0 = 0.1), 10,000 steps on the per-phase kernels: the
+    interior populations carry the SHA-256 of the reference's own SINGLE
+    and DOUBLE builds (tests/golden/make_cavity_reference.py), and FP32 differs from FP64
+    by Linf 7.4e-7 in velocity (stated tolerance 1e-5), relative L2 1.4e-5"""
+    import hashlib
+    import json
+    with open(os.path.join(HERE, "golden", "c1_reference.json")) as f:
+        ref = json.load(f)
+    cx = np.array([1, 1, 1, 0, -1, -1, -1, 0, 0])
+    cy = np.array([-1, 1, 0, 1, -1, 1, 0, -1, 0])
+    pops = {}
+    for prec in ("single", "double"):
+        s = Solver(256, 256, bc="driven", impl="original", prec=prec, svisc=0.256, u0=0.1)
+        for _ in range(10000):
+            s.boundaries()
+            s.propagation()
+            s.collision()
+        raw = s.raw()
+        s.close()
+        assert hashlib.sha256(np.ascontiguousarray(raw[:, 1:-1, 1:-1]).tobytes()).hexdigest() == \
+            ref["sha256_" + prec], prec
+        pops[prec] = raw
+
+    def fields(p):
+        f = p[:, 1:-1, 1:-1].astype(np.float64)
+        rho = f.sum(0) + 1.0
+        return rho, (cx[:, None, None] * f).sum(0) / rho, (cy[:, None, None] * f).sum(0) / rho
+    ra, ua, va = fields(pops["single"])
+    rb, ub, vb = fields(pops["double"])
+    du = np.concatenate([(ua - ub).ravel(), (va - vb).ravel()])
+    linf = np.abs(du).max()
+    rel = np.linalg.norm(du) / np.linalg.norm(np.concatenate([ub.ravel(), vb.ravel()]))
+    assert linf == pytest.approx(ref["single_vs_double"]["uv_linf"], rel=1e-9) and linf < 1e-5
+    assert rel == pytest.approx(ref["single_vs_double"]["uv_rel_l2"], rel=1e-9)

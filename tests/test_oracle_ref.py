@@ -95,3 +95,22 @@ def test_diagnostics_text(name):
     np.testing.assert_allclose(np.c_[u, v, r], pi[-lx:, 1:], rtol=6e-6, atol=1e-30)
     pr = FIX[name + "/probe"][-1]
     np.testing.assert_allclose(o.probe(lx // 2, ly // 2), pr[1:], rtol=6e-6, atol=1e-30)
+
+
+def _c1():
+    import json
+    with open(os.path.join(HERE, "golden", "c1_reference.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("prec", ["single", "double"])
+def test_c1_cavity_256_original_matches_reference_builds(prec):
+    """C1: lid-driven cavity 256^2, ORIGINAL, Re = 100, 10,000 steps: the
+    oracle's interior populations carry the same SHA-256 as the reference's
+    own SINGLE / DOUBLE builds (tests/golden/make_cavity_reference.py)"""
+    import hashlib
+    ref = _c1()
+    o = po.Oracle(256, 256, bc="driven", impl="original", prec=prec, svisc=0.256, u0=0.1)
+    o.step(10000)
+    got = hashlib.sha256(np.ascontiguousarray(o.raw()[:, 1:-1, 1:-1]).tobytes()).hexdigest()
+    assert got == ref["sha256_" + prec]
