@@ -4,7 +4,7 @@
   c1: 256^2, ORIGINAL stream + collide, Re = 100 (svisc = 0.256, u0 = 0.1),
       10,000 steps (seconds);
   c2: 4096^2, FUSED, Re = 1e4 (svisc = 0.04096, u0 = 0.1), 20,000 steps
-      (about an hour of one core for both builds).
+      (about 1.5 hours, the two builds on one core each).
 
 Stores only digests and derived numbers (<case>_reference.json): the
 SHA-256 of each build's interior populations (raw storage bits,
@@ -28,7 +28,15 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import pyoracle  # noqa: E402
 
-CASES = {"c1": ("ORIG", 256, 256, 0.256, 0.1, 10000), "c2": ("FUSED", 4096, 4096, 0.04096, 0.1, 20000)}
+# name: (build case, implementation, l, m, svisc, u0, steps, builds)
+CASES = {
+    "c1": ("LDC", "ORIG", 256, 256, 0.256, 0.1, 10000, ("SINGLE", "DOUBLE")),
+    "c2": ("LDC", "FUSED", 4096, 4096, 0.04096, 0.1, 20000, ("SINGLE", "DOUBLE")),
+    "c2short": ("LDC", "FUSED", 4096, 4096, 0.04096, 0.1, 2000, ("SINGLE", "DOUBLE")),
+    # Taylor-Green at the C3 size with the reference's own initial field
+    # (amplitude 0.1, SRC/init.F90); the SINGLE build only (DOUBLE: ~2 h)
+    "c3": ("TGV", "FUSED", 8192, 8192, 0.1, 0.1, 5000, ("SINGLE",)),
+}
 # D2Q9 in the order 01 03 05 08 10 12 14 17 19
 CX = np.array([1, 1, 1, 0, -1, -1, -1, 0, 0])
 CY = np.array([-1, 1, 0, 1, -1, 1, 0, -1, 0])
@@ -56,26 +64,35 @@ def compare(a, b):
                 rho_linf=float(np.abs(ra - rb).max()))
 
 
-def run(prec, impl, l, m, svisc, u0, steps):
-    exe = os.path.join(ROOT, "oracle", "_ref", f"ref_{prec}_{impl}_LDC")
+def run(precs, case, impl, l, m, svisc, u0, steps):
+    """the builds of `precs`, in parallel processes; their populations"""
     with tempfile.TemporaryDirectory() as td:
-        dump = os.path.join(td, "dump.bin")
-        subprocess.run([exe, str(l), str(m), repr(svisc), repr(u0), str(steps), "16", dump], cwd=td,
-                       check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
-        return pyoracle.read_ref_dump(dump)["pops"]
+        procs = []
+        for prec in precs:
+            exe = os.path.join(ROOT, "oracle", "_ref", f"ref_{prec}_{impl}_{case}")
+            wd = os.path.join(td, prec)
+            os.makedirs(wd)
+            procs.append(subprocess.Popen([exe, str(l), str(m), repr(svisc), repr(u0), str(steps), "16",
+                                           os.path.join(wd, "dump.bin")], cwd=wd,
+                                          stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL))
+        for p in procs:
+            if p.wait() != 0:
+                raise RuntimeError("reference run failed")
+        return [pyoracle.read_ref_dump(os.path.join(td, prec, "dump.bin"))["pops"] for prec in precs]
 
 
 def main():
     name = sys.argv[1] if len(sys.argv) > 1 else "c1"
-    impl, l, m, svisc, u0, steps = CASES[name]
-    s = run("SINGLE", impl, l, m, svisc, u0, steps)
-    d = run("DOUBLE", impl, l, m, svisc, u0, steps)
-    out = dict(case=f"lid-driven cavity {l}x{m}, {'ORIGINAL' if impl == 'ORIG' else impl}, "
-                    f"svisc={svisc}, u0={u0}, {steps} steps",
-               sha256_single=digest(s), sha256_double=digest(d),
-               single_vs_double=compare(s, d),
-               mass_single=float(fields(s)[0].astype(np.float64).sum()),
-               mass_double=float(fields(d)[0].sum()))
+    case, impl, l, m, svisc, u0, steps, precs = CASES[name]
+    pops = run(precs, case, impl, l, m, svisc, u0, steps)
+    title = {"LDC": "lid-driven cavity", "TGV": "Taylor-Green vortex (amplitude 0.1)"}[case]
+    out = dict(case=f"{title} {l}x{m}, {'ORIGINAL' if impl == 'ORIG' else impl}, "
+                    f"svisc={svisc}, u0={u0}, {steps} steps")
+    for prec, p in zip(precs, pops):
+        out["sha256_" + prec.lower()] = digest(p)
+        out["mass_" + prec.lower()] = float(fields(p)[0].sum())
+    if len(pops) == 2:
+        out["single_vs_double"] = compare(pops[0], pops[1])
     with open(os.path.join(HERE, f"{name}_reference.json"), "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out, indent=1))
