@@ -315,3 +315,51 @@ def test_c1_cavity_256_original_matches_reference_builds():
     rel = np.linalg.norm(du) / np.linalg.norm(np.concatenate([ub.ravel(), vb.ravel()]))
     assert linf == pytest.approx(ref["single_vs_double"]["uv_linf"], rel=1e-9) and linf < 1e-5
     assert rel == pytest.approx(ref["single_vs_double"]["uv_rel_l2"], rel=1e-9)
+
+
+def _reference_digest_case(name, bc, n, m, svisc, u0, steps, precs):
+    """run the HIP path (bgk_run, default kernels) for each storage kind
+    and compare the interior's SHA-256 with the reference build's; returns
+    the populations by kind"""
+    import hashlib
+    import json
+    with open(os.path.join(HERE, "golden", f"{name}_reference.json")) as f:
+        ref = json.load(f)
+    pops = {}
+    for prec in precs:
+        print(f"case {name} {prec}", flush=True)
+        s = Solver(n, m, bc=bc, impl="fused", prec=prec, svisc=svisc, u0=u0)
+        s.run(steps)
+        raw = s.raw()
+        s.close()
+        got = hashlib.sha256(np.ascontiguousarray(raw[:, 1:-1, 1:-1]).tobytes()).hexdigest()
+        assert got == ref["sha256_" + prec], (name, prec)
+        pops[prec] = raw
+    return ref, pops
+
+
+def _uv_diff(a, b):
+    cx = np.array([1, 1, 1, 0, -1, -1, -1, 0, 0])
+    cy = np.array([-1, 1, 0, 1, -1, 1, 0, -1, 0])
+    out = []
+    for p in (a, b):
+        f = p[:, 1:-1, 1:-1].astype(np.float64)
+        rho = f.sum(0) + 1.0
+        out.append(((cx[:, None, None] * f).sum(0) / rho, (cy[:, None, None] * f).sum(0) / rho))
+    du = np.concatenate([(out[0][0] - out[1][0]).ravel(), (out[0][1] - out[1][1]).ravel()])
+    ref = np.concatenate([out[1][0].ravel(), out[1][1].ravel()])
+    return np.abs(du).max(), np.linalg.norm(du) / np.linalg.norm(ref)
+
+
+@pytest.mark.gpu
+def test_c2_cavity_4096_fused_matches_reference_builds():
+    """C2 size: lid-driven cavity 4096^2, FUSED, Re = 1e4 (svisc = 0.04096,
+    u0 = 0.1), 2,000 steps through bgk_run (the two-step kernel): the
+    interior populations carry the SHA-256 of the reference's own SINGLE
+    and DOUBLE builds (tests/golden/make_cavity_reference.py c2short); FP32
+    against FP64 as recorded there"""
+    ref, pops = _reference_digest_case("c2short", "driven", 4096, 4096, 0.04096, 0.1, 2000,
+                                       ("single", "double"))
+    linf, rel = _uv_diff(pops["single"], pops["double"])
+    assert linf == pytest.approx(ref["single_vs_double"]["uv_linf"], rel=1e-9)
+    assert rel == pytest.approx(ref["single_vs_double"]["uv_rel_l2"], rel=1e-9)
