@@ -363,3 +363,12 @@ def test_c2_cavity_4096_fused_matches_reference_builds():
     linf, rel = _uv_diff(pops["single"], pops["double"])
     assert linf == pytest.approx(ref["single_vs_double"]["uv_linf"], rel=1e-9)
     assert rel == pytest.approx(ref["single_vs_double"]["uv_rel_l2"], rel=1e-9)
+
+
+@pytest.mark.gpu
+def test_c3_tgv_8192_fused_matches_reference_build():
+    """C3 size: Taylor-Green 8192^2, FUSED, tau = 0.8 (svisc = 0.1), the
+    reference's initial field (amplitude 0.1), 5,000 steps through bgk_run:
+    the interior populations carry the SHA-256 of the reference's own SINGLE
+    build (tests/golden/make_cavity_reference.py c3)"""
+    _reference_digest_case("c3", "periodic", 8192, 8192, 0.1, 0.1, 5000, ("single",))
