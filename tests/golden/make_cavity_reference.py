@@ -36,6 +36,9 @@ CASES = {
     # Taylor-Green at the C3 size with the reference's own initial field
     # (amplitude 0.1, SRC/init.F90); the SINGLE build only (DOUBLE: ~2 h)
     "c3": ("TGV", "FUSED", 8192, 8192, 0.1, 0.1, 5000, ("SINGLE",)),
+    # half-precision storage (the MIXED1 build: real*2 populations, real*4
+    # arithmetic) at 1024^2; the reference's real*2 runs at ~8 MLUPS here
+    "c3half": ("TGV", "FUSED", 1024, 1024, 0.1, 0.1, 500, ("MIXED1",)),
 }
 # D2Q9 in the order 01 03 05 08 10 12 14 17 19
 CX = np.array([1, 1, 1, 0, -1, -1, -1, 0, 0])
@@ -49,7 +52,8 @@ def digest(pops):
 
 def fields(pops):
     """rho, u, v in FP64 from the shifted populations (f - w)"""
-    f = pops[:, 1:-1, 1:-1].astype(np.float64)
+    f = pops[:, 1:-1, 1:-1]
+    f = (f.view(np.float16) if f.dtype == np.uint16 else f).astype(np.float64)  # real*2 bits
     rho = f.sum(0) + 1.0
     return rho, (CX[:, None, None] * f).sum(0) / rho, (CY[:, None, None] * f).sum(0) / rho
 

@@ -372,3 +372,21 @@ def test_c3_tgv_8192_fused_matches_reference_build():
     the interior populations carry the SHA-256 of the reference's own SINGLE
     build (tests/golden/make_cavity_reference.py c3)"""
     _reference_digest_case("c3", "periodic", 8192, 8192, 0.1, 0.1, 5000, ("single",))
+
+
+@pytest.mark.gpu
+def test_half_storage_tgv_1024_matches_reference_mixed1_build():
+    """FP16 storage at scale 1 in the reference's operation order (the
+    MIXED1 build: real*2 populations, real*4 arithmetic): Taylor-Green
+    1024^2, tau = 0.8, amplitude 0.1, 500 steps through bgk_run gives the
+    SHA-256 of the reference's MIXED1 build (make_cavity_reference.py
+    c3half)"""
+    import hashlib
+    import json
+    with open(os.path.join(HERE, "golden", "c3half_reference.json")) as f:
+        ref = json.load(f)
+    s = Solver(1024, 1024, bc="periodic", impl="fused", prec="fp16", svisc=0.1, u0=0.1, mixed1_quirk=1)
+    s.run(500)
+    raw = s.raw()
+    s.close()
+    assert hashlib.sha256(np.ascontiguousarray(raw[:, 1:-1, 1:-1]).tobytes()).hexdigest() == ref["sha256_mixed1"]
