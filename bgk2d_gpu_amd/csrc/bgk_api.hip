@@ -314,14 +314,27 @@ bool use_tb(const bgk_ctx *c)
 // Strip blocks of height H that must apply the obstacle rule: those whose
 // updated nodes (their strip, its rim columns, one row above and below)
 // reach the obstacle's bounding box grown by one node, where fix_pulls
-// applies it -- with one more block on each side as a margin.
+// applies it -- with one more block on each side as a margin.  Along a
+// periodic direction the first and last blocks also update wrapped nodes
+// of the far edge (their rim or halo row): when the grown box reaches an
+// edge (the obstacle touches the second or second-to-last row or column),
+// every block along that direction is taken.
 // (first strip, last strip, first row, last row) of blocks; x > y: none.
-int4 tb_box(const bgk_ctx *c, int H)
+int4 tb_box(const bgk_ctx *c, int H, bool xper, bool yper)
 {
     if (!(c->p.obstacle && c->nobs > 0)) return make_int4(0, -1, 0, -1);
     const int strips = (c->l + TB_TW - 1) / TB_TW, rows = (c->m + H - 1) / H;
-    return make_int4(max(0, (c->imin - 3) / TB_TW - 1), min(strips - 1, (c->imax + 1) / TB_TW + 1),
-                     max(0, (c->jmin - 3) / H - 1), min(rows - 1, (c->jmax + 1) / H + 1));
+    int4 b = make_int4(max(0, (c->imin - 3) / TB_TW - 1), min(strips - 1, (c->imax + 1) / TB_TW + 1),
+                       max(0, (c->jmin - 3) / H - 1), min(rows - 1, (c->jmax + 1) / H + 1));
+    if (xper && (c->imin - 1 <= 1 || c->imax + 1 >= c->l)) {
+        b.x = 0;
+        b.y = strips - 1;
+    }
+    if (yper && (c->jmin - 1 <= 1 || c->jmax + 1 >= c->m)) {
+        b.z = 0;
+        b.w = rows - 1;
+    }
+    return b;
 }
 
 // Strip height of k_fused_tb: as many strips of rows as make the grid one
@@ -374,7 +387,8 @@ int launch_tb_t(bgk_ctx *c)
     const int H = c->tb_h;
     const int strips = (c->l + TB_TW - 1) / TB_TW, rows = (c->m + H - 1) / H;
     const int rows8 = (rows + NXCD - 1) / NXCD * NXCD;
-    const int4 box = OBS ? tb_box(c, H) : make_int4(0, -1, 0, -1);  // strip blocks with the obstacle rule
+    // strip blocks with the obstacle rule
+    const int4 box = OBS ? tb_box(c, H, Per<BC>::x, Per<BC>::y) : make_int4(0, -1, 0, -1);
     if (OBS && box.y >= box.x) {
         // the obstacle's blocks on a second stream, beside the others
         // the rows of the skipped blocks, in blocks of a quarter of H

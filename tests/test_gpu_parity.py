@@ -390,3 +390,33 @@ def test_half_storage_tgv_1024_matches_reference_mixed1_build():
     raw = s.raw()
     s.close()
     assert hashlib.sha256(np.ascontiguousarray(raw[:, 1:-1, 1:-1]).tobytes()).hexdigest() == ref["sha256_mixed1"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bc", ["periodic", "channel", "inflow"])
+@pytest.mark.parametrize("edge", ["bottom", "top", "left", "right"])
+def test_two_step_obstacle_at_periodic_edges(bc, edge):
+    """a small obstacle on the second or second-to-last row or column: along
+    a periodic direction the strip blocks at the far edge update wrapped
+    nodes next to it, so they too apply the obstacle rule; the result
+    equals the one-step kernel bit for bit"""
+    l, m = 600, 260
+    mask = np.zeros((m, l), dtype=np.int32)
+    if edge == "bottom":
+        mask[1:4, 300:306] = 1      # rows 2..4
+    elif edge == "top":
+        mask[m - 4:m - 1, 100:106] = 1  # rows m-3..m-1
+    elif edge == "left":
+        mask[120:126, 1:4] = 1      # columns 2..4
+    else:
+        mask[40:46, l - 4:l - 1] = 1  # columns l-3..l-1
+    out = []
+    for tb in (1, 2):
+        print(f"case {bc} {edge} tblock={tb}", flush=True)
+        s = Solver(l, m, bc=bc, impl="fused", prec="single", svisc=0.05, u0=0.05, tblock=tb)
+        s.set_obstacle(mask)
+        s.run(41)
+        out.append(s.raw())
+        s.close()
+    a, b = out
+    assert np.array_equal(a[:, 1:-1, 1:-1].view(np.uint32), b[:, 1:-1, 1:-1].view(np.uint32))
