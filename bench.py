@@ -244,6 +244,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--serve-n", type=int, default=48)
+    ap.add_argument("--serve-full", action="store_true",
+                    help="serving leg only, on the full C5 distributions (sides 128..4096, steps 1k..50k)")
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--c4-only", action="store_true")
     ap.add_argument("--c4-steps", type=int, default=2000)
@@ -263,10 +265,13 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(each request or slab shards per GPU; see DESIGN.md)", file=sys.stderr)
+              "(a large solve would shard by y-slabs, serving by request, one GPU each)", file=sys.stderr)
         sys.exit(2)
     if args.serve_only:
         print(json.dumps(serving_leg(args.serve_n)))
+        return
+    if args.serve_full:
+        print(json.dumps(serving_leg(args.serve_n, side=(128, 4096), steps=(1000, 50000))))
         return
     if args.c4_only:
         print(json.dumps(c4_leg(args.c4_steps)))
