@@ -71,8 +71,18 @@ struct Member {
 // meta: [0, NGROUPS) task counts of the groups; [NGROUPS, 2 NGROUPS) their
 // first task in the table.  A task (member, first tile) covers the tiles
 // first .. min(first + TASK_TILES, ntiles) - 1.
+// Eight waves per SIMD asked of the compiler for the light variants (at
+// most 64 VGPRs; the 16-bit driven kernel needed 69, and spills 8 bytes
+// under the bound): 16-bit members ran 5-16 % faster.  The inflow, couette
+// and open variants would spill 28-208 bytes, so they keep the default.
+template <int BC>
+constexpr int batch_waves()
+{
+    return (BC == BK_INFLOW || BC == BK_COUETTE || BC == BK_OPEN) ? 1 : 8;
+}
+
 template <class S, int BC>
-__global__ __launch_bounds__(256) void k_batch(const Member *__restrict__ M, const int2 *__restrict__ tasks,
+__global__ __launch_bounds__(256, batch_waves<BC>()) void k_batch(const Member *__restrict__ M, const int2 *__restrict__ tasks,
                                                const int *__restrict__ meta, int group, int t)
 {
     const int n = meta[group];
