@@ -693,7 +693,7 @@ __global__ __launch_bounds__(256) void k_fused2(Lat L, Phys<C> P)
 // r+1 overwrites ring row r-3, read last by phase 2 of row r-2, which every
 // thread has finished when it passes the barrier of row r.
 constexpr int TB_TW = 256;  // strip width (one column per thread)
-constexpr int TB_RB = 32;   // rows of rim columns computed per batch (power of two)
+constexpr int TB_RB = 16;   // rows of rim columns computed per batch (power of two; 16 beat 8, 32, 64)
 
 // ring slot of population k (0..7, not 08/17): 01,03,05,10,12,14 -> 0..5
 __device__ __forceinline__ constexpr int tb_slot(int k) { return k < 3 ? k : k - 1; }

@@ -18,4 +18,4 @@ def test_profiled_traffic_is_read_from_the_profile():
     sys.path.insert(0, ROOT)
     import bench
     t = bench.profiled_traffic("k_fused_tb_fp32_ldc4096_hbm_counters.md")
-    assert t == 1336043463
+    assert t == 1329668878
