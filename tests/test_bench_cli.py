@@ -15,7 +15,9 @@ def test_more_than_one_gpu_is_refused_with_a_message():
 
 
 def test_profiled_traffic_is_read_from_the_profile():
+    """the committed counter profile holds at least the algorithmic bytes
+    of a two-step launch at 4096^2 (18 x 4 B per node) and less than twice"""
     sys.path.insert(0, ROOT)
     import bench
     t = bench.profiled_traffic("k_fused_tb_fp32_ldc4096_hbm_counters.md")
-    assert t == 1329668878
+    assert 18 * 4 * 4096 * 4096 <= t < 2 * 18 * 4 * 4096 * 4096
