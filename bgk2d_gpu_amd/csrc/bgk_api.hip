@@ -132,7 +132,7 @@ Phys<C> phys(const bgk_ctx *c)
     P.inv = (C)(1.0 / c->scale);
     for (int q = 0; q < 3; q++) P.inflow[q] = (C)c->inflow[q];
     P.umax = (C)c->umax;
-    P.channel_force = c->p.bc == BGK_BC_CHANNEL ? 1 : 0;
+    P.channel_force = c->p.bc == BGK_BC_CHANNEL ? (c->p.forcing_y ? 2 : 1) : 0;
     P.form = c->p.form;
     P.les = c->p.les;
     P.trick1 = c->p.trick1;
@@ -696,6 +696,8 @@ int bgk_create(const bgk_params_t *p, bgk_ctx_t **out)
     if ((p->les != 0 && p->les != 1) || !(p->cte_s >= 0.0)) return BGK_E_INVALID;
     // TRICK1 is the driven cavity's and, as in the reference, square only
     if (p->trick1 != 0 && (p->trick1 != 1 || p->bc != BGK_BC_DRIVEN || p->lx != p->ly)) return BGK_E_INVALID;
+    // FORCING_Y changes the CHANNEL build's forcing term only
+    if (p->forcing_y != 0 && (p->forcing_y != 1 || p->bc != BGK_BC_CHANNEL)) return BGK_E_INVALID;
     if (p->obstacle < 0 || p->obstacle > 3) return BGK_E_INVALID;
     bgk_ctx *c = new bgk_ctx();
     c->p = *p;

@@ -155,7 +155,8 @@ struct Phys {
     C sc, inv;              // storage scale and its inverse (FP16/BF16)
     C inflow[3];            // stored-and-decoded inflow equilibria 01,03,05
     C umax;                 // OPEN: peak of the parabolic inflow profile
-    int channel_force;      // 1: CHANNEL body force in collision
+    int channel_force;      // CHANNEL body force in collision: 1 along x,
+                            // 2 along y (the FORCING_Y build)
     int form;               // 0: reference operation order; 1: deviation form
     int rounding;           // FP16/BF16 stores: 0 nearest-even, 1 stochastic
     int les;                // 1: Smagorinsky LES relaxation rate per node
@@ -176,8 +177,11 @@ struct K {
 };
 
 // BGK relaxation of one node in the shifted form, operation for operation
-// as SRC/col_MC.F90:125-293 (and SRC/col.F90:104-237).
-template <class C>
+// as SRC/col_MC.F90:125-293 (and SRC/col.F90:104-237).  FORCE: the kernel
+// may serve CHANNEL, whose body force P.channel_force selects; elsewhere
+// both force terms are the compile-time zeros of the reference's
+// non-CHANNEL builds.
+template <class C, bool FORCE = true>
 __device__ __forceinline__ void relax(const Phys<C> &P, const C x[9], C o[9])
 {
     const C cte1 = (C)1.0, cte0 = (C)1.0 - cte1;
@@ -235,8 +239,11 @@ __device__ __forceinline__ void relax(const Phys<C> &P, const C x[9], C o[9])
         const C ts = (C)1.0 / ((C)2.0 * omega1) + sqrt((C)18.0 * (cs * cs) * ptot + inv1 * inv1) / (C)2.0;
         om = (C)1.0 / ts;
     }
+    // forcing term (SRC/col_MC.F90:266-274, SRC/col.F90:210-218): fgrad*rho
+    // on x, or on y in the FORCING_Y build
     C forcex = (C)0.0, forcey = (C)0.0;
-    if (P.channel_force) forcex = P.fgrad * rho;
+    if (FORCE && P.channel_force == 1) forcex = P.fgrad * rho;
+    if (FORCE && P.channel_force == 2) forcey = P.fgrad * rho;
     o[0] = x01 - om * (x01 - e01) + (+forcex - forcey);
     o[1] = x03 - om * (x03 - e03) + (+forcex + forcey);
     o[2] = x05 - om * (x05 - e05) + (+forcex);

@@ -579,7 +579,7 @@ __device__ __forceinline__ void fused_node(const Lat &L, const Phys<C> &P, int i
     for (int k = 0; k < 8; k++) x[k] = CodecT<S, C>::dec(*(const R *)(pull_base<S>(L, L.a[k], k) + bo), P.inv);
     x[8] = CodecT<S, C>::dec(*(const R *)(pull_base<S>(L, L.r, 8) + bo), P.inv);
     fix_pulls<S, C, BC, OBS>(L, P, i, j, x);
-    relax<C>(P, x, o);
+    relax<C, BC == BK_CHANNEL>(P, x, o);
 #pragma unroll
     for (int k = 0; k < 8; k++) st_nt((R *)(pull_base<S>(L, L.b[k], 8) + bo), encode<S, C>(P, o[k], i, j, k));
     st_nt((R *)(pull_base<S>(L, L.rn, 8) + bo), encode<S, C>(P, o[8], i, j, 8));
@@ -638,7 +638,7 @@ __device__ __forceinline__ void fused_pair(const Lat &L, const Phys<C> &P, int i
     for (int q = 0; q < 2; q++) {
         const int i = i0 + q;
         if (!split) fix_pulls<S, C, BC, OBS>(L, P, i, j, x[q]);
-        relax<C>(P, x[q], o);
+        relax<C, BC == BK_CHANNEL>(P, x[q], o);
 #pragma unroll
         for (int k = 0; k < 9; k++) {
             const uint32_t hb = __builtin_bit_cast(uint16_t, encode<S, C>(P, o[k], i, j, k));
@@ -725,7 +725,7 @@ __device__ __forceinline__ void tb_node(const Lat &L, const Phys<C> &P, int i, i
 #pragma unroll
     for (int k = 0; k < 9; k++) x[k] = CodecT<S, C>::dec(xr[k], P.inv);
     fix_pulls<S, C, BC, OBS>(L, P, i, j, x);
-    relax<C>(P, x, o);
+    relax<C, BC == BK_CHANNEL>(P, x, o);
 }
 
 template <class S, class R>
@@ -829,7 +829,7 @@ __device__ __forceinline__ void tb_block(const Lat &L, const Phys<C> &P, int bx,
         x[7] = CodecT<S, C>::dec(acc.h17[2], P.inv);  // 17 from (i, q+1)
         x[8] = CodecT<S, C>::dec(acc.h19[1], P.inv);
         fix_pulls_a<S, C, BC, OBS, true>(acc, L, P, i, q, x);
-        relax<C>(P, x, o);
+        relax<C, BC == BK_CHANNEL>(P, x, o);
         const uint32_t bo = ((uint32_t)q * (uint32_t)L.rs + (uint32_t)i) * (uint32_t)sizeof(S);
 #pragma unroll
         for (int k = 0; k < 8; k++) st_nt((R *)(pull_base<S>(L, L.b[k], 8) + bo), encode<S, C>(P, o[k], i, q, k));

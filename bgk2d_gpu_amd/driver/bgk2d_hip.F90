@@ -30,6 +30,7 @@
 !     les            1: Smagorinsky LES collision, as the LES build does,
 !                    with &parameters cteS
 !     trick1         1: driven-cavity corner order of the TRICK1 build
+!     forcing_y      1: POF body force along y, as the FORCING_Y build does
 !     shape          obstacle of VKS/CYL: 'CYLINDER' (default), 'SQUARE',
 !                    'HALF_CIRCLE' (the SQUARE / HALF_CIRCLE builds)
 program bgk2d_hip
@@ -42,10 +43,10 @@ program bgk2d_hip
   real(8) :: svisc, u0, cteS
   character(len=16) :: case_name, precision, implementation
   real(8) :: scale, amplitude
-  integer :: phases, device, form, rounding, obs_x, obs_y, drag, les, trick1
+  integer :: phases, device, form, rounding, obs_x, obs_y, drag, les, trick1, forcing_y
   character(len=16) :: shape
   namelist /gpu/ case_name, precision, implementation, scale, amplitude, phases, device, &
-       form, rounding, obs_x, obs_y, drag, les, trick1, shape
+       form, rounding, obs_x, obs_y, drag, les, trick1, forcing_y, shape
 
   type(bgk_params_t) :: p
   type(c_ptr) :: ctx
@@ -78,6 +79,7 @@ program bgk2d_hip
   drag = 0
   les = 0
   trick1 = 0
+  forcing_y = 0
   shape = 'CYLINDER' 
   open(15, file='bgk.input', status='old', iostat=ios)
   if (ios /= 0) error stop 'bgk.input not found'
@@ -100,6 +102,8 @@ program bgk2d_hip
   p%tblock = 0
   p%les = les
   p%trick1 = trick1
+  p%forcing_y = forcing_y
+  p%pad0 = 0
   p%obstacle = 0
   p%scale = scale
   select case (trim(case_name))

@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:6cd3fa8ba3683b1f
+﻿!mod$ v1 sum:ed38b9e32766ea62
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ bb381bf46e508468 i __fortran_builtins
 module bgk_hip
@@ -110,6 +110,8 @@ integer(4)::rounding
 integer(4)::tblock
 integer(4)::les
 integer(4)::trick1
+integer(4)::forcing_y
+integer(4)::pad0
 real(8)::svisc
 real(8)::u0
 real(8)::scale

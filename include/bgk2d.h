@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define BGK_ABI_VERSION 2
+#define BGK_ABI_VERSION 3
 
 /* boundary variant: the compile-time -DLDC/-DPERIODIC/-DCHANNEL/-DINFLOW/
  * -DCOUETTE/-DSTEP choice of SRC/boundaries.F90:31-56, made at run time  */
@@ -105,6 +105,10 @@ typedef struct bgk_params {
                              (SRC/col_MC.F90:178-230)                       */
     int32_t trick1;       /* 1: driven-cavity corner order of the TRICK1 build
                              (SRC/bcond.driven.F90:43-81; lx == ly)         */
+    int32_t forcing_y;    /* 1: CHANNEL body force along y, the FORCING_Y
+                             build (SRC/col_MC.F90:266-274)                 */
+    int32_t pad0;         /* set to 0; keeps the doubles 8-byte aligned
+                             without implicit padding                       */
     double svisc;         /* namelist svisc (kinematic viscosity)           */
     double u0;            /* namelist u0 (lid speed, or channel force scale)*/
     double scale;         /* FP16/BF16 storage scale (power of two, >= 1)   */

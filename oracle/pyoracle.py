@@ -20,9 +20,11 @@ IMPL = {"original": 0, "fused": 1}
 CASE = {"LDC": ("driven", 0), "POF": ("channel", 0), "TGV": ("periodic", 0),
         "VKS": ("inflow", 1), "COU": ("couette", 0), "BFS": ("step", 0),
         "LDCLES": ("driven", 0), "VKSLES": ("inflow", 1), "LDCT1": ("driven", 0),
-        "VKSSQ": ("inflow", 2), "VKSHC": ("inflow", 3)}
+        "VKSSQ": ("inflow", 2), "VKSHC": ("inflow", 3), "POFY": ("channel", 0)}
 # reference build cases compiled with -DTRICK1
 TRICK1_CASES = {"LDCT1"}
+# reference build cases compiled with -DFORCING_Y
+FORCING_Y_CASES = {"POFY"}
 # reference build cases compiled with -DLES (cteS = 0.1, a default-real literal)
 LES_CASES = {"LDCLES", "VKSLES"}
 CTE_S_REF = 0.10000000149011612
@@ -36,7 +38,7 @@ class OrcParams(C.Structure):
                 ("radius", C.c_int32), ("init_v", C.c_int32),
                 ("mixed1_quirk", C.c_int32), ("tgv", C.c_int32),
                 ("form", C.c_int32), ("rounding", C.c_int32),
-                ("les", C.c_int32), ("trick1", C.c_int32),
+                ("les", C.c_int32), ("trick1", C.c_int32), ("forcing_y", C.c_int32),
                 ("svisc", C.c_double), ("u0", C.c_double),
                 ("amplitude", C.c_double), ("scale", C.c_double), ("cte_s", C.c_double)]
 
@@ -102,12 +104,12 @@ class Oracle:
     def __init__(self, lx, ly, bc="driven", impl="fused", prec="single", svisc=0.1,
                  u0=0.1, obstacle=0, radius=16, amplitude=0.1, scale=1.0,
                  mixed1_quirk=0, init_v=0, tgv=None, form=0, rounding=0, les=0, cte_s=0.1,
-                 trick1=0):
+                 trick1=0, forcing_y=0):
         L = lib()
         if tgv is None:
             tgv = 1 if bc == "periodic" else 0
         self.p = OrcParams(lx, ly, BC[bc], obstacle, IMPL[impl], PREC[prec], radius,
-                           init_v, mixed1_quirk, tgv, form, rounding, les, trick1, svisc, u0, amplitude,
+                           init_v, mixed1_quirk, tgv, form, rounding, les, trick1, forcing_y, svisc, u0, amplitude,
                            scale, cte_s)
         self.prec = prec
         self.l, self.m = lx, ly

@@ -64,6 +64,10 @@ CASES = [
     ("SINGLE_ORIG_VKSSQ", 40, 24, 0.05, 0.1, 60, 4),
     ("SINGLE_FUSED_VKSHC", 40, 24, 0.05, 0.1, 60, 5),
     ("DOUBLE_FUSED_VKSHC", 40, 24, 0.05, 0.1, 60, 5),
+    # FORCING_Y: the channel's body force along y, against the walls
+    ("SINGLE_FUSED_POFY", 32, 12, 0.1, 0.1, 80, 16),
+    ("SINGLE_ORIG_POFY", 32, 12, 0.1, 0.1, 80, 16),
+    ("DOUBLE_FUSED_POFY", 32, 12, 0.1, 0.1, 80, 16),
 ]
 
 

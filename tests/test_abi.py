@@ -24,12 +24,13 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(pkg.build.LIB)
     for n in declared():
         assert hasattr(lib, n), n
-    assert lib.bgk_version() == 2
+    assert lib.bgk_version() == 3
 
 
 def test_struct_layout_matches_header():
-    # bgk_params_t: 16 int32 then 4 doubles, no padding
-    assert ctypes.sizeof(pkg.solver.BgkParams) == 16 * 4 + 4 * 8
+    # bgk_params_t: 18 int32 then 4 doubles, no padding
+    assert ctypes.sizeof(pkg.solver.BgkParams) == 18 * 4 + 4 * 8
+    assert pkg.solver.BgkParams.svisc.offset == 18 * 4
 
 
 def test_null_arguments_rejected_without_gpu():

@@ -48,7 +48,8 @@ def fixture_solver(name, impl_override=None):
     s = Solver(lx, ly, bc=bc, impl=impl, prec=GPU_PREC[p], svisc=sv, u0=u0, obstacle=obst,
                radius=rad, mixed1_quirk=1 if p == "MIXED1" else 0,
                les=int(case in po.LES_CASES), cte_s=po.CTE_S_REF,
-               trick1=int(case in po.TRICK1_CASES))
+               trick1=int(case in po.TRICK1_CASES),
+               forcing_y=int(case in po.FORCING_Y_CASES))
     return s, ns
 
 

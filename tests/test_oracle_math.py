@@ -171,18 +171,23 @@ def test_omega_one_gives_equilibrium():
                                rtol=0, atol=2e-15)
 
 
-def test_channel_force_adds_momentum_not_mass():
-    """CHANNEL forcing adds 6*fgrad*rho of x-momentum per node and no mass
-    (SRC/col.F90:212-237); checked on a uniform rest state."""
+@pytest.mark.parametrize("forcing_y", [0, 1])
+def test_channel_force_adds_momentum_not_mass(forcing_y):
+    """CHANNEL forcing adds 6*fgrad*rho of momentum per node along x (along
+    y in the FORCING_Y build), none across it and no mass
+    (SRC/col.F90:210-237); checked on a uniform rest state."""
     l, m = 8, 6
-    o = po.Oracle(l, m, bc="channel", impl="fused", prec="double", svisc=0.1, u0=0.05)
+    o = po.Oracle(l, m, bc="channel", impl="fused", prec="double", svisc=0.1, u0=0.05,
+                  forcing_y=forcing_y)
     fgrad = o.constants()["fgrad"]
     assert fgrad == pytest.approx(4 * 0.05 * 0.1 / (3 * m * m), rel=1e-15)
     o.step(1)
     f = o.pops()[:, 2:-2, 2:-2]  # away from the walls
     rho, ux, uy = moments(f)
+    along, across = (uy, ux) if forcing_y else (ux, uy)
     np.testing.assert_allclose(rho, 1.0, atol=1e-15)
-    np.testing.assert_allclose(ux * rho, 6 * fgrad, rtol=1e-12)
+    np.testing.assert_allclose(along * rho, 6 * fgrad, rtol=1e-12)
+    np.testing.assert_allclose(across, 0.0, atol=1e-18)
 
 
 def test_equilibrium_is_fixed_point():

@@ -29,16 +29,17 @@ def run_oracle(name):
     o = po.Oracle(lx, ly, bc=bc, impl="fused" if impl == "FUSED" else "original",
                   prec=PREC[p], svisc=sv, u0=u0, obstacle=obst, radius=rad,
                   mixed1_quirk=1 if p == "MIXED1" else 0, les=int(case in po.LES_CASES),
-                  cte_s=po.CTE_S_REF, trick1=int(case in po.TRICK1_CASES))
+                  cte_s=po.CTE_S_REF, trick1=int(case in po.TRICK1_CASES),
+                  forcing_y=int(case in po.FORCING_Y_CASES))
     o.step(ns)
     return o
 
 
 def test_fixture_inventory():
-    assert len({v.split("@")[0] for v in VARIANTS}) == 32
+    assert len({v.split("@")[0] for v in VARIANTS}) == 35
     cases = {v.split("@")[0].split("_")[2] for v in VARIANTS}
     assert cases == {"LDC", "TGV", "POF", "VKS", "COU", "BFS", "LDCLES", "VKSLES", "LDCT1",
-                     "VKSSQ", "VKSHC"}
+                     "VKSSQ", "VKSHC", "POFY"}
 
 
 @pytest.mark.parametrize("name", VARIANTS)
