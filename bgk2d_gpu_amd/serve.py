@@ -36,6 +36,7 @@ class Request:
     prec: str
     arrival: float = 0.0
     tblock: int = 0  # bgk_params_t.tblock: 0 automatic, 1 one step per launch, 2 two
+    forcing_y: int = 0  # bgk_params_t.forcing_y: a channel's body force along y
 
     def geometry(self):
         """(lx, ly, bc, svisc, u0, amplitude): velocity scale U = 0.1 for the
@@ -62,7 +63,8 @@ class Request:
         narrow = self.prec in ("fp16", "bf16")
         return BgkParams(lx=lx, ly=ly, bc=BC[bc], impl=IMPL["fused"], prec=PREC[self.prec],
                          radius=16, device=device, form=form if narrow else 0,
-                         rounding=rounding if narrow else 0, tblock=tblock, svisc=float(svisc),
+                         rounding=rounding if narrow else 0, tblock=tblock,
+                         forcing_y=self.forcing_y, svisc=float(svisc),
                          u0=float(u0), scale=1024.0 if narrow else 1.0, cte_s=0.1)
 
 
@@ -336,7 +338,8 @@ def serve_sequential(reqs, check_every=1000, keep=(), device=0, clock=time.perf_
         narrow = r.prec in ("fp16", "bf16")
         s = Solver(lx, ly, bc=bc, impl="fused", prec=r.prec, svisc=svisc, u0=u0, amplitude=amp,
                    scale=1024.0 if narrow else 1.0, form=1 if narrow else 0,
-                   rounding=1 if narrow else 0, tblock=r.tblock, device=device)
+                   rounding=1 if narrow else 0, tblock=r.tblock, forcing_y=r.forcing_y,
+                   device=device)
         done, status = 0, "ok"
         while done < r.steps:
             n = min(check_every, r.steps - done)

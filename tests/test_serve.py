@@ -39,6 +39,9 @@ def test_batched_members_equal_solves_alone():
             Request(3, "cavity", 97, 100.0, 333, "fp16", 0.0),
             Request(4, "tgv", 300, 1000.0, 500, "bf16", 0.0),
             Request(5, "cavity", 513, 1000.0, 250, "single", 0.0),
+            # the FORCING_Y channel (body force along y) beside the x-forced one
+            Request(9, "poiseuille", 256, 200.0, 470, "single", 0.0, forcing_y=1),
+            Request(10, "poiseuille", 300, 200.0, 390, "fp16", 0.0, forcing_y=1),
             # members of >= 2^22 nodes advance with bgk_run's kernels (the
             # two-step kernel for FP32) before the batch kernels
             Request(6, "cavity", 2050, 1000.0, 37, "single", 0.0),
@@ -66,6 +69,8 @@ def test_two_step_batch_members_equal_solves_alone():
     reqs += [Request(17, "poiseuille", 4096, 500.0, 30, "bf16", 0.0, tblock=2),
              Request(18, "poiseuille", 4000, 500.0, 27, "bf16", 0.0, tblock=2),
              Request(19, "cavity", 300, 1000.0, 50, "single", 0.0)]
+    reqs += [Request(20 + i, "poiseuille", 2048, 500.0, 44 + i, "single", 0.0, forcing_y=i & 1)
+             for i in range(4)]
     keep = {r.rid for r in reqs}
     bo, _ = serve.serve_batched(reqs, budget_bytes=60e9, chunk=10, check_every=100, keep=keep)
     so, _ = serve.serve_sequential(reqs, check_every=100, keep=keep)
