@@ -253,6 +253,10 @@ int launch_collision(bgk_ctx *c)
 
 bool foldable(const bgk_ctx *c) { return c->p.impl == BGK_IMPL_FUSED && c->p.bc != BGK_BC_STEP; }
 
+// the settings of the FAST kernel instances (16-bit storage only)
+template <class C>
+bool fast_flags(const Phys<C> &P) { return P.form == 1 && P.rounding == 1; }
+
 template <class S, class C, int BC, bool OBS>
 void launch_fused_t(bgk_ctx *c)
 {
@@ -260,7 +264,10 @@ void launch_fused_t(bgk_ctx *c)
     Phys<C> P = phys<C>(c);
     if constexpr (sizeof(S) == 2) {
         // 16-bit storage: two nodes per thread
-        hipLaunchKernelGGL((k_fused2<S, C, BC, OBS>), grid2((c->l + 1) / 2, c->m, B2), B2, 0, c->s, L, P);
+        if (fast_flags(P))
+            hipLaunchKernelGGL((k_fused2<S, C, BC, OBS, true>), grid2((c->l + 1) / 2, c->m, B2), B2, 0, c->s, L, P);
+        else
+            hipLaunchKernelGGL((k_fused2<S, C, BC, OBS>), grid2((c->l + 1) / 2, c->m, B2), B2, 0, c->s, L, P);
     } else {
         hipLaunchKernelGGL((k_fused<S, C, BC, OBS>), grid2(c->l, c->m, B2), B2, 0, c->s, L, P);
     }
@@ -348,7 +355,10 @@ int size_tb(bgk_ctx *c)
     // with an obstacle the grid is still that of the obstacle-free kernel:
     // the obstacle's few strip blocks run beside it (launch_tb_t); leaving
     // them room in the round measured slower (91 k against 94 k MLUPS)
-    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused_tb<S, C, BC, false>, 256, 0));
+    if (sizeof(S) == 2 && fast_flags(phys<C>(c)))
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused_tb<S, C, BC, false, sizeof(S) == 2>, 256, 0));
+    else
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused_tb<S, C, BC, false>, 256, 0));
     HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->dev));
     if (per_cu < 1) per_cu = 1;
     if (cus < 1) cus = 1;
@@ -384,6 +394,7 @@ int launch_tb_t(bgk_ctx *c)
     }
     Lat L = lat(c);
     Phys<C> P = phys<C>(c);
+    const bool fast = sizeof(S) == 2 && fast_flags(P);
     const int H = c->tb_h;
     const int strips = (c->l + TB_TW - 1) / TB_TW, rows = (c->m + H - 1) / H;
     const int rows8 = (rows + NXCD - 1) / NXCD * NXCD;
@@ -396,12 +407,20 @@ int launch_tb_t(bgk_ctx *c)
         const int hb = max(TB_HMIN, H / 4), nsub = (r1 - r0 + hb) / hb;
         HIPCHK(c, hipEventRecord(c->evf, c->s));
         HIPCHK(c, hipStreamWaitEvent(c->s2, c->evf, 0));
-        hipLaunchKernelGGL((k_fused_tb_box<S, C, BC>), dim3(nbx * nsub), dim3(256), 0, c->s2, L, P, box.x, nbx, r0, r1,
-                           hb);
+        if (fast)
+            hipLaunchKernelGGL((k_fused_tb_box<S, C, BC, sizeof(S) == 2>), dim3(nbx * nsub), dim3(256), 0, c->s2, L, P,
+                               box.x, nbx, r0, r1, hb);
+        else
+            hipLaunchKernelGGL((k_fused_tb_box<S, C, BC>), dim3(nbx * nsub), dim3(256), 0, c->s2, L, P, box.x, nbx, r0,
+                               r1, hb);
         HIPCHK(c, hipEventRecord(c->evj, c->s2));
     }
-    hipLaunchKernelGGL((k_fused_tb<S, C, BC, false>), dim3(strips * rows8), dim3(256), 0, c->s, L, P, H, strips, rows,
-                       box);
+    if (fast)
+        hipLaunchKernelGGL((k_fused_tb<S, C, BC, false, sizeof(S) == 2>), dim3(strips * rows8), dim3(256), 0, c->s, L, P,
+                           H, strips, rows, box);
+    else
+        hipLaunchKernelGGL((k_fused_tb<S, C, BC, false>), dim3(strips * rows8), dim3(256), 0, c->s, L, P, H, strips,
+                           rows, box);
     if (OBS && box.y >= box.x) HIPCHK(c, hipStreamWaitEvent(c->s, c->evj, 0));
     c->cur = 1 - c->cur;
     c->rcur = 1 - c->rcur;
