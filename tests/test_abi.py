@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 import bgk2d_gpu_amd as pkg
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -31,6 +33,25 @@ def test_struct_layout_matches_header():
     # bgk_params_t: 18 int32 then 4 doubles, no padding
     assert ctypes.sizeof(pkg.solver.BgkParams) == 18 * 4 + 4 * 8
     assert pkg.solver.BgkParams.svisc.offset == 18 * 4
+
+
+@pytest.mark.parametrize("change", [
+    dict(forcing_y=1, bc=0),      # FORCING_Y outside the channel
+    dict(forcing_y=2),            # not a flag
+    dict(trick1=1, bc=2),         # TRICK1 outside the driven cavity
+    dict(trick1=1, bc=0, ly=48),  # TRICK1 on a non-square cavity
+    dict(les=2), dict(form=2), dict(rounding=-1), dict(tblock=3),
+    dict(obstacle=4), dict(svisc=0.0), dict(bc=7), dict(lx=1),
+])
+def test_invalid_parameters_rejected_without_gpu(change):
+    """bgk_create checks the parameters before it touches a device"""
+    L = pkg.load()
+    kw = dict(lx=64, ly=64, bc=2, impl=1, prec=0, svisc=0.1, u0=0.1, scale=1.0, cte_s=0.1)
+    kw.update(change)
+    p = pkg.solver.BgkParams(**kw)
+    h = ctypes.c_void_p()
+    assert L.bgk_create(ctypes.byref(p), ctypes.byref(h)) == -1
+    assert not h.value
 
 
 def test_null_arguments_rejected_without_gpu():
