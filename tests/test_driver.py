@@ -75,18 +75,22 @@ def test_driver_rejects_unknown_case_before_gpu(tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("phases", [0, 1])
-@pytest.mark.parametrize("case,prec,impl", [("LDC", "SINGLE", "FUSED"), ("TGV", "DOUBLE", "ORIGINAL"),
-                                            ("VKS", "SINGLE", "FUSED"), ("POF", "MIXED1", "FUSED")])
-def test_driver_matches_oracle(tmp_path, case, prec, impl, phases):
+@pytest.mark.parametrize("case,prec,impl,extra", [("LDC", "SINGLE", "FUSED", ""),
+                                                  ("TGV", "DOUBLE", "ORIGINAL", ""),
+                                                  ("VKS", "SINGLE", "FUSED", ""),
+                                                  ("POF", "MIXED1", "FUSED", ""),
+                                                  ("POF", "SINGLE", "FUSED", "forcing_y=1")])
+def test_driver_matches_oracle(tmp_path, case, prec, impl, extra, phases):
     d = str(tmp_path)
     a = write_input(d, case=case, prec=prec, impl=impl, phases=phases, lx=80, ly=40,
-                    svisc=0.05, itfin=300, itsave=300)
+                    svisc=0.05, itfin=300, itsave=300, extra=extra)
     r = run_driver(d)
     assert r.returncode == 0, r.stdout
     bc, obst = po.CASE[case]
     oprec = {"SINGLE": "single", "DOUBLE": "double", "MIXED1": "mixed1"}[prec]
     o = po.Oracle(80, 40, bc=bc, impl=impl.lower(), prec=oprec, svisc=0.05, u0=0.1,
-                  obstacle=obst, radius=16 if not obst else 16, mixed1_quirk=int(prec == "MIXED1"))
+                  obstacle=obst, radius=16 if not obst else 16, mixed1_quirk=int(prec == "MIXED1"),
+                  forcing_y=int(extra == "forcing_y=1"))
     o.step(300)
     it, pops = read_save(os.path.join(d, "save.00000300.bin"), po.STORE_DTYPE[oprec], 80, 40)
     assert it == 300
