@@ -691,6 +691,76 @@ int fetch_red(bgk_ctx *c, int n)
     return BGK_OK;
 }
 
+// Everything a context holds, whatever part of bgk_create succeeded:
+// handles and buffers not yet made are null (the context is value-initialised).
+void ctx_release(bgk_ctx *c)
+{
+    if (c->s) (void)hipStreamSynchronize(c->s);
+    if (c->s2) (void)hipStreamSynchronize(c->s2);
+    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.second);
+    if (c->mem) (void)hipFree(c->mem);
+    if (c->d_obs) (void)hipFree(c->d_obs);
+    if (c->d_rows) (void)hipFree(c->d_rows);
+    if (c->d_red) (void)hipFree(c->d_red);
+    if (c->h_red) (void)hipHostFree(c->h_red);
+    if (c->d_stage) (void)hipFree(c->d_stage);
+    if (c->d_fields) (void)hipFree(c->d_fields);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->evf) (void)hipEventDestroy(c->evf);
+    if (c->evj) (void)hipEventDestroy(c->evj);
+    if (c->s) (void)hipStreamDestroy(c->s);
+    if (c->s2) (void)hipStreamDestroy(c->s2);
+    delete c;
+}
+
+// Message of the last context-less failure (bgk_create, bgk_footprint) on
+// this thread; bgk_last_error(NULL) returns it.
+thread_local std::string t_param_err;
+
+int param_fail(const char *why)
+{
+    t_param_err = why;
+    return BGK_E_INVALID;
+}
+
+// Largest plane the hot kernels address: they form each node's byte offset
+// within a plane in 32 bits (one VGPR per node), so a plane of (m+2) rows
+// of `pitch` elements must stay below 4 GiB.
+constexpr uint64_t PLANE_LIMIT = (uint64_t)1 << 32;
+
+// nullptr if the parameters describe a solve this library runs, else why not
+const char *param_error(const bgk_params_t *p)
+{
+    if (p->lx < 2 || p->ly < 2 || p->lx > (1 << 20) || p->ly > (1 << 20)) return "lx, ly must be in 2..2^20";
+    if (p->bc < 0 || p->bc > 6) return "unknown boundary variant";
+    if (p->prec < 0 || p->prec > 4) return "unknown precision";
+    if (p->impl != 0 && p->impl != 1) return "unknown implementation";
+    if (p->bc == BGK_BC_STEP && p->lx < 20) return "STEP needs lx >= 20 (stepx = l/20 >= 1)";
+    if (!(p->svisc > 0.0)) return "svisc must be positive";
+    if (p->form != 0 && p->form != 1) return "form must be 0 or 1";
+    if (p->rounding != 0 && p->rounding != 1) return "rounding must be 0 or 1";
+    if (p->tblock < 0 || p->tblock > 2) return "tblock must be 0, 1 or 2";
+    if ((p->les != 0 && p->les != 1) || !(p->cte_s >= 0.0)) return "les must be 0 or 1, cte_s >= 0";
+    // TRICK1 is the driven cavity's and, as in the reference, square only
+    if (p->trick1 != 0 && (p->trick1 != 1 || p->bc != BGK_BC_DRIVEN || p->lx != p->ly))
+        return "trick1 needs the square driven cavity";
+    // FORCING_Y changes the CHANNEL build's forcing term only
+    if (p->forcing_y != 0 && (p->forcing_y != 1 || p->bc != BGK_BC_CHANNEL)) return "forcing_y needs the channel";
+    if (p->obstacle < 0 || p->obstacle > 3) return "obstacle must be 0..3";
+    if (p->prec == BGK_PREC_FP16 || p->prec == BGK_PREC_BF16) {
+        int ex;
+        const double sc = p->scale > 0.0 ? p->scale : 1.0;
+        if (frexp(sc, &ex) != 0.5) return "scale must be a power of two (exact scaling)";
+    }
+    int es, A, off, pitch;
+    size_t pe;
+    layout_of(p, &es, &A, &off, &pitch, &pe);
+    if ((uint64_t)(p->ly + 2) * (uint64_t)pitch * (uint64_t)es >= PLANE_LIMIT)
+        return "lattice too large: a population plane ((ly+2) rows x pitch x element size) must stay below 4 GiB";
+    return nullptr;
+}
+
 }  // namespace
 
 // =====================================================================
@@ -698,26 +768,13 @@ extern "C" {
 
 int bgk_version(void) { return BGK_ABI_VERSION; }
 
-const char *bgk_last_error(const bgk_ctx_t *c) { return c ? c->err.c_str() : "null context"; }
+const char *bgk_last_error(const bgk_ctx_t *c) { return c ? c->err.c_str() : t_param_err.c_str(); }
 
 int bgk_create(const bgk_params_t *p, bgk_ctx_t **out)
 {
-    if (!p || !out) return BGK_E_INVALID;
+    if (!p || !out) return param_fail("null argument");
     *out = nullptr;
-    if (p->lx < 2 || p->ly < 2 || p->lx > (1 << 20) || p->ly > (1 << 20)) return BGK_E_INVALID;
-    if (p->bc < 0 || p->bc > 6 || p->prec < 0 || p->prec > 4 || (p->impl != 0 && p->impl != 1))
-        return BGK_E_INVALID;
-    if (p->bc == BGK_BC_STEP && p->lx < 20) return BGK_E_INVALID;  // stepx = l/20 must be >= 1
-    if (!(p->svisc > 0.0)) return BGK_E_INVALID;
-    if (p->form != 0 && p->form != 1) return BGK_E_INVALID;
-    if (p->rounding != 0 && p->rounding != 1) return BGK_E_INVALID;
-    if (p->tblock < 0 || p->tblock > 2) return BGK_E_INVALID;
-    if ((p->les != 0 && p->les != 1) || !(p->cte_s >= 0.0)) return BGK_E_INVALID;
-    // TRICK1 is the driven cavity's and, as in the reference, square only
-    if (p->trick1 != 0 && (p->trick1 != 1 || p->bc != BGK_BC_DRIVEN || p->lx != p->ly)) return BGK_E_INVALID;
-    // FORCING_Y changes the CHANNEL build's forcing term only
-    if (p->forcing_y != 0 && (p->forcing_y != 1 || p->bc != BGK_BC_CHANNEL)) return BGK_E_INVALID;
-    if (p->obstacle < 0 || p->obstacle > 3) return BGK_E_INVALID;
+    if (const char *why = param_error(p)) return param_fail(why);
     bgk_ctx *c = new bgk_ctx();
     c->p = *p;
     c->l = p->lx;
@@ -725,14 +782,6 @@ int bgk_create(const bgk_params_t *p, bgk_ctx_t **out)
     c->dev = p->device;
     c->scale = (p->scale > 0.0) ? p->scale : 1.0;
     if (!is_fp16(p->prec) && !is_bf16(p->prec)) c->scale = 1.0;
-    {
-        int ex;
-        double fr = frexp(c->scale, &ex);
-        if (fr != 0.5) {
-            delete c;
-            return BGK_E_INVALID;  // power of two only: scaling must be exact
-        }
-    }
     layout_of(p, &c->esize, &c->A, &c->off, &c->pitch, &c->plane_elems);
     c->imin = c->l;
     c->jmin = c->m;
@@ -750,37 +799,24 @@ int bgk_create(const bgk_params_t *p, bgk_ctx_t **out)
     c->d_fields = nullptr;
     if (hipSetDevice(c->dev) != hipSuccess) {
         delete c;
-        return BGK_E_HIP;
+        return param_fail("hipSetDevice failed");
     }
-    if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
-        return BGK_E_HIP;
+    if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
+        hipEventCreate(&c->ev1) != hipSuccess || hipEventCreateWithFlags(&c->evf, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->evj, hipEventDisableTiming) != hipSuccess) {
+        ctx_release(c);
+        return param_fail("stream or event creation failed");
     }
-    hipEventCreate(&c->ev0);
-    hipEventCreate(&c->ev1);
-    if (hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess) {
-        hipStreamDestroy(c->s);
-        delete c;
-        return BGK_E_HIP;
-    }
-    hipEventCreateWithFlags(&c->evf, hipEventDisableTiming);
-    hipEventCreateWithFlags(&c->evj, hipEventDisableTiming);
     size_t bytes = c->plane_elems * (size_t)c->esize * 18;
     size_t obs_bytes = ((size_t)c->l * c->m + 255) / 256 * 256;
     size_t rows_bytes = sizeof(double) * ROW_VALS * (size_t)c->m;
     size_t red_bytes = sizeof(double) * 3 * (size_t)(c->l > c->m ? c->l : c->m) + 64;
-    if (hipMalloc(&c->mem, bytes) != hipSuccess) {
-        hipStreamDestroy(c->s);
-        hipStreamDestroy(c->s2);
-        delete c;
-        return BGK_E_NOMEM;
-    }
-    if (hipMalloc(&c->d_obs, obs_bytes) != hipSuccess || hipMalloc(&c->d_rows, rows_bytes) != hipSuccess ||
-        hipMalloc(&c->d_red, red_bytes) != hipSuccess || hipHostMalloc(&c->h_red, red_bytes) != hipSuccess) {
-        hipFree(c->mem);
-        hipStreamDestroy(c->s);
-        hipStreamDestroy(c->s2);
-        delete c;
+    if (hipMalloc(&c->mem, bytes) != hipSuccess || hipMalloc(&c->d_obs, obs_bytes) != hipSuccess ||
+        hipMalloc(&c->d_rows, rows_bytes) != hipSuccess || hipMalloc(&c->d_red, red_bytes) != hipSuccess ||
+        hipHostMalloc(&c->h_red, red_bytes) != hipSuccess) {
+        ctx_release(c);
+        t_param_err = "device or pinned host allocation failed";
         return BGK_E_NOMEM;
     }
     for (int q = 0; q < 18; q++) c->plane[q] = (char *)c->mem + (size_t)q * c->plane_elems * c->esize;
@@ -814,23 +850,8 @@ int bgk_create(const bgk_params_t *p, bgk_ctx_t **out)
 int bgk_destroy(bgk_ctx_t *c)
 {
     if (!c) return BGK_E_INVALID;
-    hipSetDevice(c->dev);
-    hipStreamSynchronize(c->s);
-    for (auto &g : c->graphs) hipGraphExecDestroy(g.second);
-    hipFree(c->mem);
-    hipFree(c->d_obs);
-    hipFree(c->d_rows);
-    hipFree(c->d_red);
-    hipHostFree(c->h_red);
-    if (c->d_stage) hipFree(c->d_stage);
-    if (c->d_fields) hipFree(c->d_fields);
-    hipEventDestroy(c->ev0);
-    hipEventDestroy(c->ev1);
-    hipEventDestroy(c->evf);
-    hipEventDestroy(c->evj);
-    hipStreamDestroy(c->s);
-    hipStreamDestroy(c->s2);
-    delete c;
+    (void)hipSetDevice(c->dev);
+    ctx_release(c);
     return BGK_OK;
 }
 
@@ -1391,6 +1412,32 @@ int ensure_tiles(bgk_batch_t *b, size_t n)
     return BGK_OK;
 }
 
+// Everything a batch holds, whatever part of bgk_batch_create succeeded:
+// handles and buffers not yet made are null (the batch is value-initialised).
+void batch_release(bgk_batch_t *b)
+{
+    if (b->s) (void)hipStreamSynchronize(b->s);
+    for (int q = 0; q < bgk_batch_t::NQ; q++)
+        if (b->qs[q]) (void)hipStreamSynchronize(b->qs[q]);
+    drop_graphs(b);
+    for (auto &sl : b->slots)
+        if (sl.ctx) bgk_destroy(sl.ctx);
+    if (b->d_members) (void)hipFree(b->d_members);
+    if (b->h_members) (void)hipHostFree(b->h_members);
+    if (b->d_tiles) (void)hipFree(b->d_tiles);
+    if (b->h_tiles) (void)hipHostFree(b->h_tiles);
+    if (b->d_ntiles) (void)hipFree(b->d_ntiles);
+    if (b->h_ntiles) (void)hipHostFree(b->h_ntiles);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    for (int q = 0; q < bgk_batch_t::NQ; q++) {
+        if (b->qev[q]) (void)hipEventDestroy(b->qev[q]);
+        if (b->qs[q]) (void)hipStreamDestroy(b->qs[q]);
+    }
+    if (b->s) (void)hipStreamDestroy(b->s);
+    delete b;
+}
+
 }  // namespace
 
 int bgk_batch_create(int32_t device, double budget_bytes, int32_t max_members, bgk_batch_t **out)
@@ -1415,28 +1462,27 @@ int bgk_batch_create(int32_t device, double budget_bytes, int32_t max_members, b
     for (int k = 0; k < 3; k++)
         for (int v = 0; v < 6; v++) b->grid_tb[k][v] = 0;
     if (hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess) {
-        delete b;
+        batch_release(b);
         return BGK_E_HIP;
     }
     for (int q = 0; q < bgk_batch_t::NQ; q++) {
-        if (hipStreamCreateWithFlags(&b->qs[q], hipStreamNonBlocking) != hipSuccess) {
-            for (int k = 0; k < q; k++) (void)hipStreamDestroy(b->qs[k]);
-            (void)hipStreamDestroy(b->s);
-            delete b;
+        if (hipStreamCreateWithFlags(&b->qs[q], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&b->qev[q], hipEventDisableTiming) != hipSuccess) {
+            batch_release(b);
             return BGK_E_HIP;
         }
-        (void)hipEventCreateWithFlags(&b->qev[q], hipEventDisableTiming);
     }
     if (hipMalloc(&b->d_members, sizeof(bgk::Member) * max_members) != hipSuccess ||
         hipHostMalloc(&b->h_members, sizeof(bgk::Member) * max_members) != hipSuccess ||
         hipMalloc(&b->d_ntiles, sizeof(int) * 2 * bgk::NGROUPS) != hipSuccess ||
         hipHostMalloc(&b->h_ntiles, sizeof(int) * 2 * bgk::NGROUPS) != hipSuccess) {
-        (void)hipStreamDestroy(b->s);
-        delete b;
+        batch_release(b);
         return BGK_E_NOMEM;
     }
-    (void)hipEventCreate(&b->ev0);
-    (void)hipEventCreate(&b->ev1);
+    if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        batch_release(b);
+        return BGK_E_HIP;
+    }
     *out = b;
     return BGK_OK;
 }
@@ -1445,25 +1491,7 @@ int bgk_batch_destroy(bgk_batch_t *b)
 {
     if (!b) return BGK_E_INVALID;
     (void)hipSetDevice(b->dev);
-    (void)hipStreamSynchronize(b->s);
-    drop_graphs(b);
-    for (auto &sl : b->slots)
-        if (sl.ctx) bgk_destroy(sl.ctx);
-    (void)hipFree(b->d_members);
-    (void)hipHostFree(b->h_members);
-    if (b->d_tiles) (void)hipFree(b->d_tiles);
-    if (b->h_tiles) (void)hipHostFree(b->h_tiles);
-    (void)hipFree(b->d_ntiles);
-    (void)hipHostFree(b->h_ntiles);
-    (void)hipEventDestroy(b->ev0);
-    (void)hipEventDestroy(b->ev1);
-    for (int q = 0; q < bgk_batch_t::NQ; q++) {
-        (void)hipStreamSynchronize(b->qs[q]);
-        (void)hipEventDestroy(b->qev[q]);
-        (void)hipStreamDestroy(b->qs[q]);
-    }
-    (void)hipStreamDestroy(b->s);
-    delete b;
+    batch_release(b);
     return BGK_OK;
 }
 
@@ -1471,7 +1499,8 @@ const char *bgk_batch_last_error(const bgk_batch_t *b) { return b ? b->err.c_str
 
 int bgk_footprint(const bgk_params_t *p, double *bytes)
 {
-    if (!p || !bytes || p->lx < 2 || p->ly < 2 || p->prec < 0 || p->prec > 4) return BGK_E_INVALID;
+    if (!p || !bytes) return param_fail("null argument");
+    if (const char *why = param_error(p)) return param_fail(why);
     *bytes = (double)footprint_of(p);
     return BGK_OK;
 }
@@ -1485,6 +1514,7 @@ int bgk_batch_add(bgk_batch_t *b, const bgk_params_t *p, int32_t init_v, double 
         !(p->prec == BGK_PREC_SINGLE || p->prec == BGK_PREC_FP16 || p->prec == BGK_PREC_BF16))
         return bfail(b, BGK_E_INVALID, "batch members are FUSED, without obstacle or step, fp32/fp16/bf16");
     if (p->device != b->dev) return bfail(b, BGK_E_INVALID, "member on device %d, batch on %d", p->device, b->dev);
+    if (const char *why = param_error(p)) return bfail(b, BGK_E_INVALID, "%s", why);
     int tw_log2, th;
     if (!tile_shape(tile, &tw_log2, &th)) return bfail(b, BGK_E_INVALID, "unknown tile code %d", tile);
     const size_t bytes = footprint_of(p);

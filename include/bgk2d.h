@@ -121,9 +121,16 @@ typedef struct bgk_ctx bgk_ctx_t;
 /* Context: allocates the two population sets in HBM (18 pitched planes:
  * eight moving populations per set and two planes of the rest population),
  * fills them with huge(mykind) as alloca does, and derives omega, fgrad
- * and the lid speed in the compute kind exactly as hencol/build_bcond. */
+ * and the lid speed in the compute kind exactly as hencol/build_bcond.
+ * Size limit: one plane, (ly+2) rows of pitch = roundup(lx+3+A, A)
+ * elements (A = 128 bytes / element size), must stay below 4 GiB, because
+ * the hot kernels form per-node byte offsets in 32 bits: about 32,700^2
+ * nodes in FP32, 23,100^2 in FP64, 46,300^2 in FP16/BF16.  Larger
+ * requests fail with BGK_E_INVALID before any device call.              */
 int bgk_create(const bgk_params_t *p, bgk_ctx_t **out);
 int bgk_destroy(bgk_ctx_t *ctx);
+/* message of the context's last failure; with ctx == NULL, of the last
+ * failed bgk_create or bgk_footprint on the calling thread               */
 const char *bgk_last_error(const bgk_ctx_t *ctx);
 int bgk_version(void);
 
@@ -209,7 +216,8 @@ typedef struct bgk_batch bgk_batch_t;
 int bgk_batch_create(int32_t device, double budget_bytes, int32_t max_members, bgk_batch_t **out);
 int bgk_batch_destroy(bgk_batch_t *b);
 const char *bgk_batch_last_error(const bgk_batch_t *b);
-/* device bytes a context with these parameters allocates */
+/* device bytes a context with these parameters allocates; BGK_E_INVALID
+ * for parameters bgk_create would refuse (the same checks)               */
 int bgk_footprint(const bgk_params_t *p, double *bytes);
 /* create + init a member that will run nsteps steps; tile: 0 default,
  * 1 = 64x4, 2 = 128x2, 3 = 256x1, 4 = 32x8 nodes per block.  Returns

@@ -63,3 +63,31 @@ def test_null_arguments_rejected_without_gpu():
                     ("bgk_batch_run", (None, 10, None)), ("bgk_batch_destroy", (None,)),
                     ("bgk_draglift", (None, 1, 2, 1, 2, None)), ("bgk_fields", (None, None, None, None))):
         assert getattr(L, f)(*args) == -1, f
+
+
+@pytest.mark.parametrize("lx,ly,prec", [(32768, 32768, 0), (24000, 24000, 1), (46400, 46400, 2)])
+def test_lattice_beyond_32bit_plane_offsets_rejected_without_gpu(lx, ly, prec):
+    """The hot kernels address a population plane with 32-bit byte offsets:
+    bgk_create, bgk_footprint and bgk_batch_add refuse a plane of 4 GiB or
+    more (32768^2 FP32, 24000^2 FP64, 46400^2 FP16) before any device call,
+    with the reason in bgk_last_error(NULL)."""
+    L = pkg.load()
+    p = pkg.solver.BgkParams(lx=lx, ly=ly, bc=0, impl=1, prec=prec, svisc=0.1, u0=0.1, scale=1.0,
+                             cte_s=0.1)
+    h = ctypes.c_void_p()
+    assert L.bgk_create(ctypes.byref(p), ctypes.byref(h)) == -1
+    assert not h.value
+    L.bgk_last_error.restype = ctypes.c_char_p
+    assert b"4 GiB" in L.bgk_last_error(None)
+    d = ctypes.c_double(0)
+    assert L.bgk_footprint(ctypes.byref(p), ctypes.byref(d)) == -1
+
+
+def test_largest_admissible_plane_has_a_footprint():
+    """just below the limit the footprint is given (no device needed)"""
+    L = pkg.load()
+    p = pkg.solver.BgkParams(lx=32000, ly=32000, bc=0, impl=1, prec=0, svisc=0.1, u0=0.1, scale=1.0,
+                             cte_s=0.1)
+    d = ctypes.c_double(0)
+    assert L.bgk_footprint(ctypes.byref(p), ctypes.byref(d)) == 0
+    assert 18 * 32000 * 32000 * 4 < d.value < 18 * 32002 * 32064 * 4 + 32000 * 32000 + 2 ** 24
