@@ -49,6 +49,36 @@ def gpu_solve(n, prec, bc, steps, warmup, svisc, u0, tblock=0):
     return dict(wall=wall, kernel_ms=kernel_ms, ok=ok, diag=diag.tolist())
 
 
+def fp16_leg(steps):
+    """FP16 shifted storage against FP32: Taylor-Green 8192^2 (tau = 0.8,
+    u0 = 0.01), each with one and with two steps per pass; the ratio of the
+    best of each"""
+    n3 = 8192
+    fp = {}
+    for prec in ("fp16", "single"):
+        for tb in (1, 2):
+            r = gpu_solve(n3, prec, "periodic", steps, 50, 0.1, 0.0, tblock=tb)
+            fp[(prec, tb)] = (n3 * n3 * steps / r["wall"] / 1e6, r)
+    ml16 = max(fp[("fp16", 1)][0], fp[("fp16", 2)][0])
+    ml32 = max(fp[("single", 1)][0], fp[("single", 2)][0])
+    return {"workload": "Taylor-Green 8192x8192 periodic, tau=0.8, u0=0.01, "
+                        f"{steps} steps, scale=1024, deviation form, stochastic rounding",
+            "mlups_fp16": round(ml16, 1), "mlups_fp32": round(ml32, 1),
+            "ratio": round(ml16 / ml32, 3),
+            "one_step_per_pass": {
+                "mlups_fp16": round(fp[("fp16", 1)][0], 1),
+                "mlups_fp32": round(fp[("single", 1)][0], 1),
+                "ratio": round(fp[("fp16", 1)][0] / fp[("single", 1)][0], 3),
+                "hbm_frac_fp16": round(36 * n3 * n3 / (fp[("fp16", 1)][1]["kernel_ms"] / 1e3)
+                                       / 1e9 / HBM_PEAK_GBS, 4),
+                "kernel_ms_per_step_fp16": round(fp[("fp16", 1)][1]["kernel_ms"], 5)},
+            "two_steps_per_pass": {
+                "mlups_fp16": round(fp[("fp16", 2)][0], 1),
+                "mlups_fp32": round(fp[("single", 2)][0], 1),
+                "ratio": round(fp[("fp16", 2)][0] / fp[("single", 2)][0], 3),
+                "kernel_ms_per_step_fp16": round(fp[("fp16", 2)][1]["kernel_ms"], 5)}}
+
+
 def cpu_baseline(budget_s=20.0):
     """The reference's own FUSED SINGLE LDC sources built serially by flang
     (oracle/_ref), on a bounded sample of the same 4096^2 workload."""
@@ -245,6 +275,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp16-steps", type=int, default=2000)
     ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--fp16-only", action="store_true",
+                    help="only the FP16/FP32 Taylor-Green 8192^2 leg (one and two steps per pass)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--serve-n", type=int, default=48)
@@ -280,6 +312,9 @@ def main():
     if args.c4_only:
         print(json.dumps(c4_leg(args.c4_steps)))
         return
+    if args.fp16_only:
+        print(json.dumps(fp16_leg(args.fp16_steps)))
+        return
 
     n = 4096
     # the FP32 solve as bgk_run does it: two FUSED steps per pass over HBM
@@ -302,16 +337,7 @@ def main():
            "traffic_bytes_per_launch": profiled_traffic("k_fused_fp32_ldc4096_hbm_counters.md")}
 
     if not args.no_fp16:
-        # FP16 shifted storage: Taylor-Green 8192^2 (tau = 0.8, u0 = 0.01) vs FP32,
-        # each with one and with two steps per pass
-        n3 = 8192
-        fp = {}
-        for prec in ("fp16", "single"):
-            for tb in (1, 2):
-                r = gpu_solve(n3, prec, "periodic", args.fp16_steps, 50, 0.1, 0.0, tblock=tb)
-                fp[(prec, tb)] = (n3 * n3 * args.fp16_steps / r["wall"] / 1e6, r)
-        ml16 = max(fp[("fp16", 1)][0], fp[("fp16", 2)][0])
-        ml32 = max(fp[("single", 1)][0], fp[("single", 2)][0])
+        fp16 = fp16_leg(args.fp16_steps)
 
     out = {
         "metric": METRIC,
@@ -345,21 +371,7 @@ def main():
         "healthy": bool(r32["ok"] and r1["ok"]),
     }
     if not args.no_fp16:
-        out["fp16_store"] = {"workload": "Taylor-Green 8192x8192 periodic, tau=0.8, u0=0.01, "
-                                         f"{args.fp16_steps} steps, scale=1024, deviation form, "
-                                         "stochastic rounding",
-                             "mlups_fp16": round(ml16, 1), "mlups_fp32": round(ml32, 1),
-                             "ratio": round(ml16 / ml32, 3),
-                             "one_step_per_pass": {
-                                 "mlups_fp16": round(fp[("fp16", 1)][0], 1),
-                                 "mlups_fp32": round(fp[("single", 1)][0], 1),
-                                 "ratio": round(fp[("fp16", 1)][0] / fp[("single", 1)][0], 3),
-                                 "hbm_frac_fp16": round(36 * n3 * n3 / (fp[("fp16", 1)][1]["kernel_ms"] / 1e3)
-                                                        / 1e9 / HBM_PEAK_GBS, 4)},
-                             "two_steps_per_pass": {
-                                 "mlups_fp16": round(fp[("fp16", 2)][0], 1),
-                                 "mlups_fp32": round(fp[("single", 2)][0], 1),
-                                 "ratio": round(fp[("fp16", 2)][0] / fp[("single", 2)][0], 3)}}
+        out["fp16_store"] = fp16
     if not args.no_c4:
         out["cylinder_channel"] = c4_leg(args.c4_steps)
     if not args.no_serve:

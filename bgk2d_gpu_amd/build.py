@@ -16,14 +16,18 @@ FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 
 # -ffp-contract=off: every product and sum separately rounded, matching the
 # reference's x86-64 build bit for bit.  Packed FP32 ops are disabled.
-HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
              "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-Wno-unused-value", "-Wno-unused-command-line-argument"]
 
-SOURCES = [os.path.join(PKG, "csrc", "bgk_api.hip")]
-DEPS = SOURCES + [os.path.join(PKG, "csrc", f) for f in ("bgk_device.hpp", "bgk_kernels.hpp",
-                                                          "bgk_serve.hpp")
-                  if os.path.exists(os.path.join(PKG, "csrc", f))] + \
+CSRC = os.path.join(PKG, "csrc")
+# the C ABI, then the step loop of each storage kind and the batch kernels
+# (explicit instantiations, compiled in parallel)
+SOURCES = [os.path.join(CSRC, f) for f in ("bgk_api.hip", "bgk_kind_f32.hip", "bgk_kind_f64.hip",
+                                           "bgk_kind_f16.hip", "bgk_kind_bf16.hip", "bgk_kind_mixed2.hip",
+                                           "bgk_kind_batch.hip")]
+DEPS = SOURCES + [os.path.join(CSRC, f) for f in ("bgk_device.hpp", "bgk_kernels.hpp", "bgk_serve.hpp",
+                                                  "bgk_host.hpp")] + \
     [os.path.join(ROOT, "include", "bgk2d.h")]
 
 
@@ -35,10 +39,18 @@ def _run(cmd, cwd=None):
     return r.stdout
 
 
-def build_lib(verbose=False):
-    os.makedirs(LIB_DIR, exist_ok=True)
+def build_lib(verbose=False, jobs=None):
+    """compile every translation unit (in parallel), then link"""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(LIB_DIR, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    objs = [os.path.join(objdir, os.path.basename(src)[:-4] + ".o") for src in SOURCES]
+    jobs = jobs or min(len(SOURCES), max(1, os.cpu_count() or 1))
+    with ThreadPoolExecutor(jobs) as ex:
+        for f in [ex.submit(_run, [HIPCC] + HIP_FLAGS + ["-c", "-o", o, src]) for src, o in zip(SOURCES, objs)]:
+            f.result()
     tmp = LIB + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + ["-o", tmp] + SOURCES)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

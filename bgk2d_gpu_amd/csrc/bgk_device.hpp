@@ -40,16 +40,13 @@ struct bf16_t {
 // Stochastic rounding of FP16/BF16 stores (opt-in): the bits below the
 // target precision get a pseudo-random offset before truncation, which makes
 // the rounding unbiased; with round-to-nearest, slowly varying fields lose
-// their sub-ulp per-step changes systematically.  The offset is a hash of
-// the value's own bits and its position (i, j, population), so results are
-// deterministic and independent of launch grouping.  The conversion itself
-// is gfx950's v_cvt_sr_f16_f32 / v_cvt_sr_bf16_f32: it adds the top 13
+// their sub-ulp per-step changes systematically.  The offset is derived from
+// the value's own bits and its position (i, j) and population k, so results
+// are deterministic and independent of launch grouping.  The conversion
+// itself is gfx950's v_cvt_sr_f16_f32 / v_cvt_sr_bf16_f32: it adds the top 13
 // (FP16) or 16 (BF16) bits of its random operand to the magnitude below the
-// kept bits (aligned to the top of the dropped bits for half subnormals)
-// and truncates; NaN stays NaN, overflow gives inf.  One instruction in
-// place of the range checks, add, mask and conversion.
-// position salt: 24-bit multiplies (full rate on CDNA) and a Weyl step per
-// population; the per-store hash below does the mixing
+// kept bits (aligned to the top of the dropped bits for half subnormals) and
+// truncates; NaN stays NaN, overflow gives inf.
 __host__ __device__ inline uint32_t mul24(uint32_t a, uint32_t b)
 {
 #ifdef __HIP_DEVICE_COMPILE__
@@ -58,16 +55,24 @@ __host__ __device__ inline uint32_t mul24(uint32_t a, uint32_t b)
     return (a & 0xffffffu) * (b & 0xffffffu);
 #endif
 }
-__host__ __device__ inline uint32_t salt_of(int i, int j, int k)
+// position salt of node (i, j): two 24-bit multiplies (full rate on CDNA),
+// computed once per node
+__host__ __device__ inline uint32_t salt_of(int i, int j)
 {
-    return mul24((uint32_t)i, 0x9E3779u) + mul24((uint32_t)j, 0x85EBCAu) + (uint32_t)k * 0x9E3779B9u;
+    return mul24((uint32_t)i, 0x9E3779u) + mul24((uint32_t)j, 0x85EBCAu);
 }
-// per-store random word: one 24-bit multiply (full rate on CDNA) of the
-// low bits of value^salt, which change from step to step; the conversion
-// takes its top bits, the best-mixed bits of the product
-__host__ __device__ inline uint32_t sr_word(uint32_t bits, uint32_t salt)
+// odd 24-bit multiplier of population k's random word
+__host__ __device__ constexpr uint32_t sr_mul(int k)
 {
-    return mul24(bits ^ salt, 0xB5297Au);
+    return k == 0 ? 0xB5297Bu : k == 1 ? 0x9E3779u : k == 2 ? 0xC2B2AFu : k == 3 ? 0x85EBCBu : k == 4 ? 0xD35A2Du
+         : k == 5 ? 0xA3B195u : k == 6 ? 0xE6546Bu : k == 7 ? 0x8B1E5Fu : 0xF1357Bu;
+}
+// per-store random word: one 24-bit multiply-add (v_mad_u32_u24) of the low
+// bits of the stored value, which change from step to step, and the node's
+// salt; the conversion takes its top bits
+__host__ __device__ inline uint32_t sr_word(uint32_t bits, uint32_t salt, int k)
+{
+    return mul24(bits, sr_mul(k)) + salt;
 }
 
 typedef __fp16 sr_half2 __attribute__((ext_vector_type(2)));
@@ -80,33 +85,56 @@ template <>
 struct Codec<float, float> {
     using raw = float;
     __device__ __forceinline__ static float dec(float v, float) { return v; }
-    __device__ __forceinline__ static float enc(float v, float, int, uint32_t) { return v; }
+    __device__ __forceinline__ static float enc(float v, float, int, uint32_t, int) { return v; }
 };
 template <>
 struct Codec<double, double> {
     using raw = double;
     __device__ __forceinline__ static double dec(double v, double) { return v; }
-    __device__ __forceinline__ static double enc(double v, double, int, uint32_t) { return v; }
+    __device__ __forceinline__ static double enc(double v, double, int, uint32_t, int) { return v; }
 };
 template <>
 struct Codec<float, double> {  // MIXED2
     using raw = float;
     __device__ __forceinline__ static double dec(float v, double) { return (double)v; }
-    __device__ __forceinline__ static float enc(double v, double, int, uint32_t) { return (float)v; }
+    __device__ __forceinline__ static float enc(double v, double, int, uint32_t, int) { return (float)v; }
 };
 template <>
 struct Codec<_Float16, float> {
     using raw = _Float16;
     // dec/enc take the scale factor that applies: inv_scale / scale
     __device__ __forceinline__ static float dec(_Float16 v, float inv) { return (float)v * inv; }
-    __device__ __forceinline__ static _Float16 enc(float v, float sc, int sr, uint32_t salt)
+    __device__ __forceinline__ static _Float16 enc(float v, float sc, int sr, uint32_t salt, int k)
     {
         const float x = v * sc;
         if (sr) {
-            const sr_half2 h = __builtin_amdgcn_cvt_sr_f16_f32(sr_half2{}, x, sr_word(__float_as_uint(x), salt), false);
+            const sr_half2 h = __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(sr_half2, x), x,
+                                                               sr_word(__float_as_uint(x), salt, k), false);
             return (_Float16)h[0];
         }
         return (_Float16)x;
+    }
+    // two values stored as the halves of one dword: enc_lo converts the
+    // first into the low half (the high half is left undefined with
+    // stochastic rounding, zero otherwise), enc_hi the second into the high
+    // half of that word in place
+    __device__ __forceinline__ static uint32_t enc_lo(float v, float sc, int sr, uint32_t salt, int k)
+    {
+        const float x = v * sc;
+        if (sr)
+            return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(sr_half2, x), x,
+                                                                                sr_word(__float_as_uint(x), salt, k),
+                                                                                false));
+        return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)x);
+    }
+    __device__ __forceinline__ static uint32_t enc_hi(uint32_t w, float v, float sc, int sr, uint32_t salt, int k)
+    {
+        const float x = v * sc;
+        if (sr)
+            return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(sr_half2, w), x,
+                                                                                sr_word(__float_as_uint(x), salt, k),
+                                                                                true));
+        return w | ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)x) << 16);
     }
 };
 template <>
@@ -116,21 +144,43 @@ struct Codec<bf16_t, float> {
     {
         return __uint_as_float((uint32_t)v.bits << 16) * inv;
     }
-    __device__ __forceinline__ static bf16_t enc(float v, float sc, int sr, uint32_t salt)
+    __device__ __forceinline__ static uint16_t rne(float f)
+    {
+        uint32_t x = __float_as_uint(f);
+        if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);
+        x += 0x7fffu + ((x >> 16) & 1u);
+        return (uint16_t)(x >> 16);
+    }
+    __device__ __forceinline__ static bf16_t enc(float v, float sc, int sr, uint32_t salt, int k)
     {
         const float f = v * sc;
-        uint32_t x = __float_as_uint(f);
         bf16_t r;
         if (sr) {
-            const sr_short2 h = __builtin_amdgcn_cvt_sr_bf16_f32(sr_short2{}, f, sr_word(x, salt), false);
+            const sr_short2 h = __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(sr_short2, f), f,
+                                                                 sr_word(__float_as_uint(f), salt, k), false);
             r.bits = (uint16_t)h[0];
-        } else if ((x & 0x7fffffffu) > 0x7f800000u) {
-            r.bits = (uint16_t)((x >> 16) | 0x40u);
         } else {
-            x += 0x7fffu + ((x >> 16) & 1u);
-            r.bits = (uint16_t)(x >> 16);
+            r.bits = rne(f);
         }
         return r;
+    }
+    __device__ __forceinline__ static uint32_t enc_lo(float v, float sc, int sr, uint32_t salt, int k)
+    {
+        const float x = v * sc;
+        if (sr)
+            return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(sr_short2, x), x,
+                                                                                 sr_word(__float_as_uint(x), salt, k),
+                                                                                 false));
+        return (uint32_t)rne(x);
+    }
+    __device__ __forceinline__ static uint32_t enc_hi(uint32_t w, float v, float sc, int sr, uint32_t salt, int k)
+    {
+        const float x = v * sc;
+        if (sr)
+            return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(sr_short2, w), x,
+                                                                                 sr_word(__float_as_uint(x), salt, k),
+                                                                                 true));
+        return w | ((uint32_t)rne(x) << 16);
     }
 };
 
@@ -162,6 +212,13 @@ struct Phys {
     int les;                // 1: Smagorinsky LES relaxation rate per node
     int trick1;             // 1: DRIVEN top corners take the side walls' copy
     C ctes;                 // LES: Smagorinsky constant cteS
+    // LES terms that depend on omega and cteS only, evaluated once on the
+    // host in the compute kind with the reference's operations:
+    // 1/(2 omega), (1/omega)^2, 18 cteS^2
+    C les_h, les_i2, les_c;
+    // kernels that compute on scaled populations (scale*(f - w) as stored,
+    // see fixed_flags): the storage scale and its inverse; 1 otherwise
+    C usc, dsc;
 };
 
 // ---------------------------------------------------------- arithmetic
@@ -177,13 +234,28 @@ struct K {
 };
 
 // BGK relaxation of one node in the shifted form, operation for operation
-// as SRC/col_MC.F90:125-293 (and SRC/col.F90:104-237).  FORCE: the kernel
-// may serve CHANNEL, whose body force P.channel_force selects; elsewhere
-// both force terms are the compile-time zeros of the reference's
-// non-CHANNEL builds.
-template <class C, bool FORCE = true>
+// as SRC/col_MC.F90:125-293 (and SRC/col.F90:104-237).
+//   FORCE : the kernel may serve CHANNEL, whose body force P.channel_force
+//           selects; without it the force terms of the reference's
+//           non-CHANNEL builds are zeros.
+//   FL    : -1 the form is P.form; 0 or 1 that form, fixed at compile time.
+//   SCALED: x and o are the populations as stored, scale*(f - w) (P.usc =
+//           scale, P.dsc = 1/scale; form 1 only).  Scaling by a power of two
+//           commutes with every rounding here, so each result is exactly
+//           scale times the unscaled one: the store needs no multiply.
+//   Z0    : storage kind = compute kind.  No stored population is then ever
+//           -0 (a difference a - b is -0 only for a = -0), so adding the
+//           +0 force of a non-CHANNEL build changes nothing and is left out.
+// The cte0 + q terms of the four non-rest equilibria are left out too: q is
+// qf*(3a*a - vsq) with 3a*a >= +0, never -0, so 0 + q = q.  The rest
+// population's q0 = cte0 + qf*(-vsq) keeps it (-vsq is -0 at rest).
+// Form 1 (deviation form, an extension) has no force terms at all outside
+// CHANNEL.
+template <class C, bool FORCE = true, int FL = -1, bool SCALED = false, bool Z0 = false>
 __device__ __forceinline__ void relax(const Phys<C> &P, const C x[9], C o[9])
 {
+    static_assert(!SCALED || FL == 1, "scaled populations need the deviation form");
+    const int form = FL >= 0 ? FL : P.form;
     const C cte1 = (C)1.0, cte0 = (C)1.0 - cte1;
     const C rf = K<C>::rf, qf = K<C>::qf, p0 = K<C>::p0, p1 = K<C>::p1, p2 = K<C>::p2;
     const C x01 = x[0], x03 = x[1], x05 = x[2], x08 = x[3], x10 = x[4];
@@ -191,28 +263,31 @@ __device__ __forceinline__ void relax(const Phys<C> &P, const C x[9], C o[9])
     // deviation form: drho = sum of the shifted populations is kept apart
     // from the 1, and the equilibria use w*drho instead of (rho*w - w), so
     // the O(u^2) density deviation is not rounded against 1
-    const C drho = ((x01 + x03 + x10 + x12) + (x05 + x08 + x14 + x17)) + x19;
-    C rho = P.form ? (C)1.0 + drho : drho + cte1;
+    const C drs = ((x01 + x03 + x10 + x12) + (x05 + x08 + x14 + x17)) + x19;  // scaled when SCALED
+    const C drho = SCALED ? drs * P.dsc : drs;
+    C rho = form ? (C)1.0 + drho : drho + cte1;
     C rhoinv = (C)1.0 / rho;
-    C vx = ((x03 - x12) + (x01 - x10) + (x05 - x14)) * rhoinv;
-    C vy = ((x03 - x01) + (x12 - x10) + (x08 - x17)) * rhoinv;
+    const C rinv = SCALED ? rhoinv * P.dsc : rhoinv;
+    C vx = ((x03 - x12) + (x01 - x10) + (x05 - x14)) * rinv;
+    C vy = ((x03 - x01) + (x12 - x10) + (x08 - x17)) * rinv;
     C vx2 = vx * vx, vy2 = vy * vy;
     C vsq = vx2 + vy2;
     C vxpy = vx + vy, vxmy = vx - vy;
-    C qxpy = cte0 + qf * ((C)3.0 * vxpy * vxpy - vsq);
-    C qxmy = cte0 + qf * ((C)3.0 * vxmy * vxmy - vsq);
-    C qx = cte0 + qf * ((C)3.0 * vx2 - vsq);
-    C qy = cte0 + qf * ((C)3.0 * vy2 - vsq);
+    C qxpy = qf * ((C)3.0 * vxpy * vxpy - vsq);
+    C qxmy = qf * ((C)3.0 * vxmy * vxmy - vsq);
+    C qx = qf * ((C)3.0 * vx2 - vsq);
+    C qy = qf * ((C)3.0 * vy2 - vsq);
     C q0 = cte0 + qf * (-vsq);
     vx = rf * vx;
     vy = rf * vy;
     vxpy = rf * vxpy;
     vxmy = rf * vxmy;
-    C rp0 = rho * p0, rp1 = rho * p1, rp2 = rho * p2;
+    const C rhoe = SCALED ? rho * P.usc : rho;
+    C rp0 = rhoe * p0, rp1 = rhoe * p1, rp2 = rhoe * p2;
     // the density term cte1*(rp - p) of the reference, or p*drho
-    const C d0 = P.form ? p0 * drho : cte1 * (rp0 - p0);
-    const C d1 = P.form ? p1 * drho : cte1 * (rp1 - p1);
-    const C d2 = P.form ? p2 * drho : cte1 * (rp2 - p2);
+    const C d0 = form ? p0 * drs : cte1 * (rp0 - p0);
+    const C d1 = form ? p1 * drs : cte1 * (rp1 - p1);
+    const C d2 = form ? p2 * drs : cte1 * (rp2 - p2);
     C e01 = rp2 * (+vxmy + qxmy) + d2;
     C e03 = rp2 * (+vxpy + qxpy) + d2;
     C e05 = rp1 * (+vx + qx) + d1;
@@ -233,26 +308,41 @@ __device__ __forceinline__ void relax(const Phys<C> &P, const C x[9], C o[9])
         const C pyy = n01 + n03 + n08 + n10 + n12 + n17;
         const C pxy = -n01 + n03 + n10 - n12;
         const C pyx = pxy;
-        const C ptot = sqrt(pxx * pxx + (C)2.0 * pxy * pyx + pyy * pyy);
-        const C omega1 = P.omega, cs = P.ctes;
-        const C inv1 = (C)1.0 / omega1;
-        const C ts = (C)1.0 / ((C)2.0 * omega1) + sqrt((C)18.0 * (cs * cs) * ptot + inv1 * inv1) / (C)2.0;
+        C ptot = sqrt(pxx * pxx + (C)2.0 * pxy * pyx + pyy * pyy);
+        if (SCALED) ptot = ptot * P.dsc;
+        // Ts = 1/(2 omega1) + sqrt(18 cteS^2 P + (1/omega1)^2) / 2
+        const C ts = P.les_h + sqrt(P.les_c * ptot + P.les_i2) / (C)2.0;
         om = (C)1.0 / ts;
     }
-    // forcing term (SRC/col_MC.F90:266-274, SRC/col.F90:210-218): fgrad*rho
-    // on x, or on y in the FORCING_Y build
-    C forcex = (C)0.0, forcey = (C)0.0;
-    if (FORCE && P.channel_force == 1) forcex = P.fgrad * rho;
-    if (FORCE && P.channel_force == 2) forcey = P.fgrad * rho;
-    o[0] = x01 - om * (x01 - e01) + (+forcex - forcey);
-    o[1] = x03 - om * (x03 - e03) + (+forcex + forcey);
-    o[2] = x05 - om * (x05 - e05) + (+forcex);
-    o[3] = x08 - om * (x08 - e08) + (+forcey);
-    o[4] = x10 - om * (x10 - e10) + (-forcex - forcey);
-    o[5] = x12 - om * (x12 - e12) + (-forcex + forcey);
-    o[6] = x14 - om * (x14 - e14) + (-forcex);
-    o[7] = x17 - om * (x17 - e17) + (-forcey);
+    C t[9];
+    t[0] = x01 - om * (x01 - e01);
+    t[1] = x03 - om * (x03 - e03);
+    t[2] = x05 - om * (x05 - e05);
+    t[3] = x08 - om * (x08 - e08);
+    t[4] = x10 - om * (x10 - e10);
+    t[5] = x12 - om * (x12 - e12);
+    t[6] = x14 - om * (x14 - e14);
+    t[7] = x17 - om * (x17 - e17);
     o[8] = x19 - om * (x19 - e19);
+    const bool fon = FORCE && P.channel_force != 0;
+    if (fon || (form == 0 && !Z0)) {
+        // forcing term (SRC/col_MC.F90:266-274, SRC/col.F90:210-218):
+        // fgrad*rho on x, or on y in the FORCING_Y build
+        C forcex = (C)0.0, forcey = (C)0.0;
+        if (FORCE && P.channel_force == 1) forcex = P.fgrad * rhoe;
+        if (FORCE && P.channel_force == 2) forcey = P.fgrad * rhoe;
+        o[0] = t[0] + (+forcex - forcey);
+        o[1] = t[1] + (+forcex + forcey);
+        o[2] = t[2] + (+forcex);
+        o[3] = t[3] + (+forcey);
+        o[4] = t[4] + (-forcex - forcey);
+        o[5] = t[5] + (-forcex + forcey);
+        o[6] = t[6] + (-forcex);
+        o[7] = t[7] + (-forcey);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; k++) o[k] = t[k];
+    }
 }
 
 // Shifted equilibrium at rho = 1 with the quadratic part in double

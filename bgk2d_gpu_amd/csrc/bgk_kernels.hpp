@@ -16,6 +16,8 @@
 #pragma once
 #include "bgk_device.hpp"
 
+#include <type_traits>
+
 namespace bgk {
 
 template <class S, class C>
@@ -36,7 +38,22 @@ __device__ __forceinline__ C get(const Lat &L, const Phys<C> &P, void *const *se
 template <class S, class C>
 __device__ __forceinline__ typename CodecT<S, C>::raw encode(const Phys<C> &P, C v, int i, int j, int k)
 {
-    return CodecT<S, C>::enc(v, P.sc, P.rounding, P.rounding ? salt_of(i, j, k) : 0u);
+    return CodecT<S, C>::enc(v, P.sc, P.rounding, P.rounding ? salt_of(i, j) : 0u, k);
+}
+// the same with the node's salt already formed
+template <class S, class C>
+__device__ __forceinline__ typename CodecT<S, C>::raw encode_s(const Phys<C> &P, C v, uint32_t salt, int k)
+{
+    return CodecT<S, C>::enc(v, P.sc, P.rounding, salt, k);
+}
+// A fresh copy of a 32-bit value for the compiler: an address formed from
+// it after the kernel's branches stays a scalar base plus this 32-bit vector
+// offset (the saddr form of global loads and stores) instead of a 64-bit
+// vector address per access.  An empty asm statement; no instruction.
+__device__ __forceinline__ uint32_t fresh(uint32_t x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
 }
 template <class S, class C>
 __device__ __forceinline__ void put(const Lat &L, const Phys<C> &P, void *const *set, int k, int i, int j, C v)
@@ -553,29 +570,55 @@ __device__ __forceinline__ void fix_pulls(const Lat &L, const Phys<C> &P, int i,
     fix_pulls_a<S, C, BC, OBS>(HbmAcc<S, C, BC>{L, P}, L, P, i, j, x);
 }
 
-// FAST: a kernel instance for 16-bit storage in the deviation form with
-// stochastic rounding, the usual settings; those two fields of P become
-// constants, so the compiler drops the other form's density terms and the
-// nearest-even store: 7 % fewer instructions in the paired and the two-step
-// kernels, at the same occupancy.  The host picks it only for those
-// settings, so the values are those of the generic instance.  (One kernel
-// holding both paths under a uniform branch needed 66-90 VGPRs instead of
-// 45-74.)
-template <class C, bool FAST>
+// Compile-time configuration of a hot-kernel instance.  FL: -1 the generic
+// instance (form and rounding read from Phys); 0 the reference's form with
+// nearest-even stores; 1 the deviation form with stochastic rounding of
+// 16-bit stores.  The host picks 0 or 1 when the parameters are those, so
+// the values are those of the generic instance; fixing them lets the
+// compiler drop the other form's density terms and store path.  With FL = 1
+// and 16-bit storage, the variants whose folded boundary rules are copies
+// and the lid term (not the INFLOW/OPEN equilibria) compute on the
+// populations as stored (SCALED: no multiply per load or store).
+template <class S, class C, int BC, int FL>
+struct Cfg {
+    static constexpr bool FORCE = BC == BK_CHANNEL;
+    static constexpr bool SCALED = FL == 1 && sizeof(S) == 2 &&
+                                   (BC == BK_DRIVEN || BC == BK_PERIODIC || BC == BK_CHANNEL || BC == BK_COUETTE);
+    static constexpr bool Z0 = std::is_same<S, C>::value;
+};
+
+template <class S, class C, int BC, int FL>
 __device__ __forceinline__ Phys<C> fixed_flags(const Phys<C> &P)
 {
     Phys<C> Q = P;
-    if constexpr (FAST) {
-        Q.form = 1;
-        Q.rounding = 1;
+    if constexpr (FL >= 0) {
+        Q.form = FL;
+        Q.rounding = FL;  // only the 16-bit codecs read it
+    }
+    if constexpr (Cfg<S, C, BC, FL>::SCALED) {
+        // populations stay scale*(f - w) in registers: codecs without the
+        // scale, the lid term scaled once (exact: a power of two)
+        Q.usc = P.sc;
+        Q.dsc = P.inv;
+        Q.sc = (C)1.0;
+        Q.inv = (C)1.0;
+        Q.force = P.force * P.sc;
     }
     return Q;
+}
+
+template <class S, class C, int BC, int FL>
+__device__ __forceinline__ void relax_k(const Phys<C> &P, const C x[9], C o[9])
+{
+    using K = Cfg<S, C, BC, FL>;
+    relax<C, K::FORCE, FL, K::SCALED, K::Z0>(P, x, o);
 }
 
 // Addressing of the hot kernel: one 32-bit byte offset per node, and per
 // population a uniform base pointer that already includes the population's
 // pull shift, so every load and store is scalar base + vector offset with
-// no per-access address arithmetic (planes are smaller than 4 GiB).
+// no per-access address arithmetic (planes are smaller than 4 GiB, which
+// bgk_create enforces).
 template <class S>
 __device__ __forceinline__ const char *pull_base(const Lat &L, const void *plane, int k)
 {
@@ -586,11 +629,11 @@ __device__ __forceinline__ const char *pull_base(const Lat &L, const void *plane
 // store into the other set.  The rest population goes to the second pop-19
 // plane rather than in place: the OPEN inflow reads the rest population of
 // the neighbouring rows' first column, which must still be the old one.
-template <class S, class C, int BC, bool OBS>
+template <class S, class C, int BC, bool OBS, int FL>
 __device__ __forceinline__ void fused_node(const Lat &L, const Phys<C> &P, int i, int j)
 {
     using R = typename CodecT<S, C>::raw;
-    const uint32_t bo = ((uint32_t)j * (uint32_t)L.rs + (uint32_t)i) * (uint32_t)sizeof(S);
+    const uint32_t bo = fresh(((uint32_t)j * (uint32_t)L.rs + (uint32_t)i) * (uint32_t)sizeof(S));
     C x[9], o[9];
     // plain pulls; the halo ring is allocated, so edge nodes read it too
     // and then replace those values below
@@ -598,19 +641,21 @@ __device__ __forceinline__ void fused_node(const Lat &L, const Phys<C> &P, int i
     for (int k = 0; k < 8; k++) x[k] = CodecT<S, C>::dec(*(const R *)(pull_base<S>(L, L.a[k], k) + bo), P.inv);
     x[8] = CodecT<S, C>::dec(*(const R *)(pull_base<S>(L, L.r, 8) + bo), P.inv);
     fix_pulls<S, C, BC, OBS>(L, P, i, j, x);
-    relax<C, BC == BK_CHANNEL>(P, x, o);
+    relax_k<S, C, BC, FL>(P, x, o);
+    const uint32_t bs = fresh(bo);
+    const uint32_t salt = P.rounding ? salt_of(i, j) : 0u;
 #pragma unroll
-    for (int k = 0; k < 8; k++) st_nt((R *)(pull_base<S>(L, L.b[k], 8) + bo), encode<S, C>(P, o[k], i, j, k));
-    st_nt((R *)(pull_base<S>(L, L.rn, 8) + bo), encode<S, C>(P, o[8], i, j, 8));
+    for (int k = 0; k < 8; k++) st_nt((R *)(pull_base<S>(L, L.b[k], 8) + bs), encode_s<S, C>(P, o[k], salt, k));
+    st_nt((R *)(pull_base<S>(L, L.rn, 8) + bs), encode_s<S, C>(P, o[8], salt, 8));
 }
 
-template <class S, class C, int BC, bool OBS>
+template <class S, class C, int BC, bool OBS, int FL = -1>
 __global__ __launch_bounds__(256) void k_fused(Lat L, Phys<C> P)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     if (i > L.l || j > L.m) return;
-    fused_node<S, C, BC, OBS>(L, P, i, j);
+    fused_node<S, C, BC, OBS, FL>(L, fixed_flags<S, C, BC, FL>(P), i, j);
 }
 
 // The same step for 16-bit storage, two nodes (i0, i0+1) per thread with
@@ -618,7 +663,9 @@ __global__ __launch_bounds__(256) void k_fused(Lat L, Phys<C> P)
 // memory instructions of the 16-bit kinds.  Node i0 starts a dword (the
 // first interior node is 128-byte aligned); the x-shifted pulls start at
 // an odd element and are assembled from the two aligned dwords around them.
-template <class S, class C, int BC, bool OBS>
+// The results of node i0 are converted into the low halves of the stored
+// dwords, then node i0+1's into the high halves in place.
+template <class S, class C, int BC, bool OBS, int FL>
 __device__ __forceinline__ void fused_pair(const Lat &L, const Phys<C> &P, int i0, int j)
 {
     static_assert(sizeof(S) == 2, "pairs of 16-bit elements");
@@ -626,16 +673,19 @@ __device__ __forceinline__ void fused_pair(const Lat &L, const Phys<C> &P, int i
     // code inlined twice below would raise the register count of every pair
     // (to ~150): there, pairs that touch an edge or the obstacle take the
     // one-node path instead.  The other variants keep it inline, which is
-    // faster for them (the obstacle rule included).
+    // faster for them (the obstacle rule included).  A last node without a
+    // partner (odd l) takes the one-node path in every variant.
     constexpr bool split = BC == BK_INFLOW || BC == BK_OPEN;
-    if (split && (i0 == 1 || i0 + 1 >= L.l || j == 1 || j == L.m ||
-                  (OBS && i0 + 1 >= L.imin - 1 && i0 <= L.imax + 1 && j >= L.jmin - 1 && j <= L.jmax + 1))) {
-        fused_node<S, C, BC, OBS>(L, P, i0, j);
-        if (i0 + 1 <= L.l) fused_node<S, C, BC, OBS>(L, P, i0 + 1, j);
+    if (i0 == L.l || (split && (i0 == 1 || i0 + 1 >= L.l || j == 1 || j == L.m ||
+                                (OBS && i0 + 1 >= L.imin - 1 && i0 <= L.imax + 1 && j >= L.jmin - 1 &&
+                                 j <= L.jmax + 1)))) {
+        fused_node<S, C, BC, OBS, FL>(L, P, i0, j);
+        if (i0 + 1 <= L.l) fused_node<S, C, BC, OBS, FL>(L, P, i0 + 1, j);
         return;
     }
     using R = typename CodecT<S, C>::raw;
-    const uint32_t bo = ((uint32_t)j * (uint32_t)L.rs + (uint32_t)i0) * 2u;
+    using Cd = CodecT<S, C>;
+    const uint32_t bo = fresh(((uint32_t)j * (uint32_t)L.rs + (uint32_t)i0) * 2u);
     C x[2][9], o[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
@@ -648,39 +698,33 @@ __device__ __forceinline__ void fused_pair(const Lat &L, const Phys<C> &P, int i
             const uint32_t lo = *(const uint32_t *)(base + bo - 2), hi = *(const uint32_t *)(base + bo + 2);
             w = (lo >> 16) | (hi << 16);
         }
-        x[0][k] = CodecT<S, C>::dec(__builtin_bit_cast(R, (uint16_t)(w & 0xffffu)), P.inv);
-        x[1][k] = CodecT<S, C>::dec(__builtin_bit_cast(R, (uint16_t)(w >> 16)), P.inv);
+        x[0][k] = Cd::dec(__builtin_bit_cast(R, (uint16_t)(w & 0xffffu)), P.inv);
+        x[1][k] = Cd::dec(__builtin_bit_cast(R, (uint16_t)(w >> 16)), P.inv);
     }
     uint32_t ow[9];
-    const bool two = i0 + 1 <= L.l;
+    const uint32_t s0 = P.rounding ? salt_of(i0, j) : 0u;
+    const uint32_t s1 = s0 + 0x9E3779u;  // salt_of(i0 + 1, j)
+    if (!split) fix_pulls<S, C, BC, OBS>(L, P, i0, j, x[0]);
+    relax_k<S, C, BC, FL>(P, x[0], o);
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
-        const int i = i0 + q;
-        if (!split) fix_pulls<S, C, BC, OBS>(L, P, i, j, x[q]);
-        relax<C, BC == BK_CHANNEL>(P, x[q], o);
+    for (int k = 0; k < 9; k++) ow[k] = Cd::enc_lo(o[k], P.sc, P.rounding, s0, k);
+    if (!split) fix_pulls<S, C, BC, OBS>(L, P, i0 + 1, j, x[1]);
+    relax_k<S, C, BC, FL>(P, x[1], o);
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-            const uint32_t hb = __builtin_bit_cast(uint16_t, encode<S, C>(P, o[k], i, j, k));
-            ow[k] = q == 0 ? hb : (ow[k] | (hb << 16));
-        }
-    }
+    for (int k = 0; k < 9; k++) ow[k] = Cd::enc_hi(ow[k], o[k], P.sc, P.rounding, s1, k);
+    const uint32_t bs = fresh(bo);
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-        char *dst = (char *)pull_base<S>(L, k == 8 ? L.rn : L.b[k], 8) + bo;
-        if (two)
-            __builtin_nontemporal_store(ow[k], (uint32_t *)dst);
-        else
-            __builtin_nontemporal_store((uint16_t)(ow[k] & 0xffffu), (uint16_t *)dst);
-    }
+    for (int k = 0; k < 9; k++)
+        __builtin_nontemporal_store(ow[k], (uint32_t *)((char *)pull_base<S>(L, k == 8 ? L.rn : L.b[k], 8) + bs));
 }
 
-template <class S, class C, int BC, bool OBS, bool FAST = false>
+template <class S, class C, int BC, bool OBS, int FL = -1>
 __global__ __launch_bounds__(256) void k_fused2(Lat L, Phys<C> P)
 {
     const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * 2 + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     if (i0 > L.l || j > L.m) return;
-    fused_pair<S, C, BC, OBS>(L, fixed_flags<C, FAST>(P), i0, j);
+    fused_pair<S, C, BC, OBS, FL>(L, fixed_flags<S, C, BC, FL>(P), i0, j);
 }
 
 // ------------------------------------------------ two steps per HBM pass
@@ -736,7 +780,7 @@ struct TbAcc {
 };
 
 // step t+1 of interior node (i, j) from raw pulls, all nine populations
-template <class S, class C, int BC, bool OBS>
+template <class S, class C, int BC, bool OBS, int FL>
 __device__ __forceinline__ void tb_node(const Lat &L, const Phys<C> &P, int i, int j,
                                         const typename CodecT<S, C>::raw xr[9], C o[9])
 {
@@ -744,20 +788,20 @@ __device__ __forceinline__ void tb_node(const Lat &L, const Phys<C> &P, int i, i
 #pragma unroll
     for (int k = 0; k < 9; k++) x[k] = CodecT<S, C>::dec(xr[k], P.inv);
     fix_pulls<S, C, BC, OBS>(L, P, i, j, x);
-    relax<C, BC == BK_CHANNEL>(P, x, o);
+    relax_k<S, C, BC, FL>(P, x, o);
 }
 
 template <class S, class R>
 __device__ __forceinline__ void tb_load(const Lat &L, int i, int j, R xr[9])
 {
-    const uint32_t bo = ((uint32_t)j * (uint32_t)L.rs + (uint32_t)i) * (uint32_t)sizeof(S);
+    const uint32_t bo = fresh(((uint32_t)j * (uint32_t)L.rs + (uint32_t)i) * (uint32_t)sizeof(S));
 #pragma unroll
     for (int k = 0; k < 8; k++) xr[k] = *(const R *)(pull_base<S>(L, L.a[k], k) + bo);
     xr[8] = *(const R *)(pull_base<S>(L, L.r, 8) + bo);
 }
 
 // the strip block of strip bx (columns bx*TB_TW+1..) and rows j0..jend
-template <class S, class C, int BC, bool OBS>
+template <class S, class C, int BC, bool OBS, int FL>
 __device__ __forceinline__ void tb_block(const Lat &L, const Phys<C> &P, int bx, int j0, int jend)
 {
     using R = typename CodecT<S, C>::raw;
@@ -802,11 +846,12 @@ __device__ __forceinline__ void tb_block(const Lat &L, const Phys<C> &P, int bx,
                 R yr[9];
                 C o[9];
                 tb_load<S>(L, wi, wj, yr);
-                tb_node<S, C, BC, OBS>(L, P, wi, wj, yr, o);
+                tb_node<S, C, BC, OBS, FL>(L, P, wi, wj, yr, o);
+                const uint32_t salt = P.rounding ? salt_of(wi, wj) : 0u;
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
                     const int kk = side ? k + 4 : k;
-                    rim[side][rr][k] = encode<S, C>(P, o[kk], wi, wj, kk);
+                    rim[side][rr][k] = encode_s<S, C>(P, o[kk], salt, kk);
                 }
             }
             __syncthreads();
@@ -820,13 +865,14 @@ __device__ __forceinline__ void tb_block(const Lat &L, const Phys<C> &P, int bx,
         }
         if (own && wr >= 0) {
             C o[9];
-            tb_node<S, C, BC, OBS>(L, P, i, wr, xr, o);
+            tb_node<S, C, BC, OBS, FL>(L, P, i, wr, xr, o);
+            const uint32_t salt = P.rounding ? salt_of(i, wr) : 0u;
 #pragma unroll
             for (int k = 0; k < 8; k++)
-                if (cx_of(k) != 0) ring[slot][tb_slot(k)][e] = encode<S, C>(P, o[k], i, wr, k);
-            acc.h08[2] = encode<S, C>(P, o[3], i, wr, 3);
-            acc.h17[2] = encode<S, C>(P, o[7], i, wr, 7);
-            acc.h19[2] = encode<S, C>(P, o[8], i, wr, 8);
+                if (cx_of(k) != 0) ring[slot][tb_slot(k)][e] = encode_s<S, C>(P, o[k], salt, k);
+            acc.h08[2] = encode_s<S, C>(P, o[3], salt, 3);
+            acc.h17[2] = encode_s<S, C>(P, o[7], salt, 7);
+            acc.h19[2] = encode_s<S, C>(P, o[8], salt, 8);
         }
         if (tid < 6) {
             const int side = tid >= 3 ? 1 : 0, k = tid - side * 3;
@@ -848,11 +894,12 @@ __device__ __forceinline__ void tb_block(const Lat &L, const Phys<C> &P, int bx,
         x[7] = CodecT<S, C>::dec(acc.h17[2], P.inv);  // 17 from (i, q+1)
         x[8] = CodecT<S, C>::dec(acc.h19[1], P.inv);
         fix_pulls_a<S, C, BC, OBS, true>(acc, L, P, i, q, x);
-        relax<C, BC == BK_CHANNEL>(P, x, o);
-        const uint32_t bo = ((uint32_t)q * (uint32_t)L.rs + (uint32_t)i) * (uint32_t)sizeof(S);
+        relax_k<S, C, BC, FL>(P, x, o);
+        const uint32_t bo = fresh(((uint32_t)q * (uint32_t)L.rs + (uint32_t)i) * (uint32_t)sizeof(S));
+        const uint32_t salt = P.rounding ? salt_of(i, q) : 0u;
 #pragma unroll
-        for (int k = 0; k < 8; k++) st_nt((R *)(pull_base<S>(L, L.b[k], 8) + bo), encode<S, C>(P, o[k], i, q, k));
-        st_nt((R *)(pull_base<S>(L, L.rn, 8) + bo), encode<S, C>(P, o[8], i, q, 8));
+        for (int k = 0; k < 8; k++) st_nt((R *)(pull_base<S>(L, L.b[k], 8) + bo), encode_s<S, C>(P, o[k], salt, k));
+        st_nt((R *)(pull_base<S>(L, L.rn, 8) + bo), encode_s<S, C>(P, o[8], salt, 8));
     }
 }
 
@@ -873,13 +920,13 @@ __device__ __forceinline__ bool tb_xcd_block(int b, int strips, int rows, int &b
 
 // box = (first, last strip; first, last row of strip blocks): blocks left
 // to k_fused_tb_box (an empty range, x > y, skips none)
-template <class S, class C, int BC, bool OBS, bool FAST = false>
+template <class S, class C, int BC, bool OBS, int FL = -1>
 __global__ __launch_bounds__(256) void k_fused_tb(Lat L, Phys<C> P, int H, int strips, int rows, int4 box)
 {
     int bx, by;
     if (!tb_xcd_block((int)blockIdx.x, strips, rows, bx, by)) return;
     if (bx >= box.x && bx <= box.y && by >= box.z && by <= box.w) return;
-    tb_block<S, C, BC, OBS>(L, fixed_flags<C, FAST>(P), bx, by * H + 1, min(by * H + H, L.m));
+    tb_block<S, C, BC, OBS, FL>(L, fixed_flags<S, C, BC, FL>(P), bx, by * H + 1, min(by * H + H, L.m));
 }
 
 // The strip blocks that reach the obstacle, with the obstacle rule: strips
@@ -888,11 +935,12 @@ __global__ __launch_bounds__(256) void k_fused_tb(Lat L, Phys<C> P, int H, int s
 // the register count (to ~100) and the time per node, so only these blocks
 // carry it, shorter than the others so that they end with them, and the
 // grid of the others keeps its occupancy.
-template <class S, class C, int BC, bool FAST = false>
+template <class S, class C, int BC, int FL = -1>
 __global__ __launch_bounds__(256) void k_fused_tb_box(Lat L, Phys<C> P, int bx0, int nbx, int r0, int r1, int hb)
 {
     const int j0 = r0 + ((int)blockIdx.x / nbx) * hb;
-    tb_block<S, C, BC, true>(L, fixed_flags<C, FAST>(P), bx0 + (int)blockIdx.x % nbx, j0, min(j0 + hb - 1, r1));
+    tb_block<S, C, BC, true, FL>(L, fixed_flags<S, C, BC, FL>(P), bx0 + (int)blockIdx.x % nbx, j0,
+                                 min(j0 + hb - 1, r1));
 }
 
 // ------------------------------------------------------------ diagnostics
@@ -957,6 +1005,7 @@ __global__ __launch_bounds__(ROWS_T) void k_rows(Lat L, Phys<C> P, double *out)
 
 // sums of the per-row values in a fixed order: out = {Srho, Su, Sv, Ss,
 // max s, nonfinite}
+template <int = 0>
 __global__ __launch_bounds__(ROWS_T) void k_rows_final(const double *rows, int m, double *out)
 {
     double acc[6] = {0, 0, 0, 0, 0, 0};

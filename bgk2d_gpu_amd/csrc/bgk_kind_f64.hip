@@ -1,0 +1,8 @@
+// bgk_kind_f64.hip -- the step loop (every hot kernel) of one storage kind,
+// instantiated apart so the library compiles in parallel
+#include "bgk_host.hpp"
+
+namespace bgk_host {
+template int run_t<double, double>(bgk_ctx *, int);
+template int enqueue_t<double, double>(bgk_ctx *, int);
+}  // namespace bgk_host

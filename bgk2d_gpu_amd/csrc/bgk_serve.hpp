@@ -128,9 +128,9 @@ __global__ __launch_bounds__(256, batch_waves<BC>()) void k_batch(const Member *
             if (i > L.l || j > L.m) continue;
             // one node (fp32) or the pair (i, i+1) (16-bit kinds), as bgk_run's kernels
             if constexpr (sizeof(S) == 2)
-                fused_pair<S, float, BC, false>(L, P, i, j);
+                fused_pair<S, float, BC, false, -1>(L, P, i, j);
             else
-                fused_node<S, float, BC, false>(L, P, i, j);
+                fused_node<S, float, BC, false, -1>(L, P, i, j);
         }
     }
 }
@@ -154,7 +154,7 @@ __device__ __forceinline__ T cld(const T *p)
 }
 __device__ __forceinline__ Phys<float> cld_phys(const Phys<float> *p)
 {
-    static_assert(sizeof(Phys<float>) == 15 * 4, "cld_phys copies every field of Phys");
+    static_assert(sizeof(Phys<float>) == 20 * 4, "cld_phys copies every field of Phys");
     Phys<float> r;
     r.omega = cld(&p->omega);
     r.fgrad = cld(&p->fgrad);
@@ -170,6 +170,11 @@ __device__ __forceinline__ Phys<float> cld_phys(const Phys<float> *p)
     r.les = cld(&p->les);
     r.trick1 = cld(&p->trick1);
     r.ctes = cld(&p->ctes);
+    r.les_h = cld(&p->les_h);
+    r.les_i2 = cld(&p->les_i2);
+    r.les_c = cld(&p->les_c);
+    r.usc = cld(&p->usc);
+    r.dsc = cld(&p->dsc);
     return r;
 }
 
@@ -210,7 +215,7 @@ __global__ __launch_bounds__(256) void k_batch_tb(const Member *__restrict__ M, 
         const Phys<float> P = cld_phys(&d->P);
         const int tiles_x = cld(&d->tiles_x);
         const int th = cld(&d->th), j0 = (tile / tiles_x) * th + 1;
-        tb_block<S, float, BC, false>(L, P, tile % tiles_x, j0, min(j0 + th - 1, L.m));
+        tb_block<S, float, BC, false, -1>(L, P, tile % tiles_x, j0, min(j0 + th - 1, L.m));
     }
 }
 
