@@ -233,43 +233,49 @@ struct K {
     static constexpr C p2 = (C)(1.0 / (3.0 * 3.0 * (1.0 + 3.0)));
 };
 
-// BGK relaxation of one node in the shifted form, operation for operation
-// as SRC/col_MC.F90:125-293 (and SRC/col.F90:104-237).
+// Smagorinsky LES rate (SRC/col_MC.F90:178-230, SRC/col.F90:154-205): the
+// relaxation rate from the non-equilibrium stress of populations x with
+// equilibria e; omega1 is the molecular rate of hencol.  With SCALED inputs
+// the stress is rescaled once (exact, a power of two).
+template <class C, bool SCALED>
+__device__ __forceinline__ C les_rate(const Phys<C> &P, const C x[9], const C e[9])
+{
+    const C n01 = x[0] - e[0], n03 = x[1] - e[1], n05 = x[2] - e[2], n08 = x[3] - e[3];
+    const C n10 = x[4] - e[4], n12 = x[5] - e[5], n14 = x[6] - e[6], n17 = x[7] - e[7];
+    const C pxx = n01 + n03 + n05 + n10 + n12 + n14;
+    const C pyy = n01 + n03 + n08 + n10 + n12 + n17;
+    const C pxy = -n01 + n03 + n10 - n12;
+    const C pyx = pxy;
+    C ptot = sqrt(pxx * pxx + (C)2.0 * pxy * pyx + pyy * pyy);
+    if (SCALED) ptot = ptot * P.dsc;
+    // Ts = 1/(2 omega1) + sqrt(18 cteS^2 P + (1/omega1)^2) / 2
+    const C ts = P.les_h + sqrt(P.les_c * ptot + P.les_i2) / (C)2.0;
+    return (C)1.0 / ts;
+}
+
+// BGK relaxation of one node, form 0: the shifted form operation for
+// operation as SRC/col_MC.F90:125-293 (and SRC/col.F90:104-237), bit-exact
+// with the reference's builds.
 //   FORCE : the kernel may serve CHANNEL, whose body force P.channel_force
 //           selects; without it the force terms of the reference's
 //           non-CHANNEL builds are zeros.
-//   FL    : -1 the form is P.form; 0 or 1 that form, fixed at compile time.
-//   SCALED: x and o are the populations as stored, scale*(f - w) (P.usc =
-//           scale, P.dsc = 1/scale; form 1 only).  Scaling by a power of two
-//           commutes with every rounding here, so each result is exactly
-//           scale times the unscaled one: the store needs no multiply.
 //   Z0    : storage kind = compute kind.  No stored population is then ever
 //           -0 (a difference a - b is -0 only for a = -0), so adding the
 //           +0 force of a non-CHANNEL build changes nothing and is left out.
 // The cte0 + q terms of the four non-rest equilibria are left out too: q is
 // qf*(3a*a - vsq) with 3a*a >= +0, never -0, so 0 + q = q.  The rest
 // population's q0 = cte0 + qf*(-vsq) keeps it (-vsq is -0 at rest).
-// Form 1 (deviation form, an extension) has no force terms at all outside
-// CHANNEL.
-template <class C, bool FORCE = true, int FL = -1, bool SCALED = false, bool Z0 = false>
-__device__ __forceinline__ void relax(const Phys<C> &P, const C x[9], C o[9])
+template <class C, bool FORCE, bool Z0>
+__device__ __forceinline__ void relax_ref(const Phys<C> &P, const C x[9], C o[9])
 {
-    static_assert(!SCALED || FL == 1, "scaled populations need the deviation form");
-    const int form = FL >= 0 ? FL : P.form;
     const C cte1 = (C)1.0, cte0 = (C)1.0 - cte1;
     const C rf = K<C>::rf, qf = K<C>::qf, p0 = K<C>::p0, p1 = K<C>::p1, p2 = K<C>::p2;
     const C x01 = x[0], x03 = x[1], x05 = x[2], x08 = x[3], x10 = x[4];
     const C x12 = x[5], x14 = x[6], x17 = x[7], x19 = x[8];
-    // deviation form: drho = sum of the shifted populations is kept apart
-    // from the 1, and the equilibria use w*drho instead of (rho*w - w), so
-    // the O(u^2) density deviation is not rounded against 1
-    const C drs = ((x01 + x03 + x10 + x12) + (x05 + x08 + x14 + x17)) + x19;  // scaled when SCALED
-    const C drho = SCALED ? drs * P.dsc : drs;
-    C rho = form ? (C)1.0 + drho : drho + cte1;
+    C rho = (((x01 + x03 + x10 + x12) + (x05 + x08 + x14 + x17)) + x19) + cte1;
     C rhoinv = (C)1.0 / rho;
-    const C rinv = SCALED ? rhoinv * P.dsc : rhoinv;
-    C vx = ((x03 - x12) + (x01 - x10) + (x05 - x14)) * rinv;
-    C vy = ((x03 - x01) + (x12 - x10) + (x08 - x17)) * rinv;
+    C vx = ((x03 - x12) + (x01 - x10) + (x05 - x14)) * rhoinv;
+    C vy = ((x03 - x01) + (x12 - x10) + (x08 - x17)) * rhoinv;
     C vx2 = vx * vx, vy2 = vy * vy;
     C vsq = vx2 + vy2;
     C vxpy = vx + vy, vxmy = vx - vy;
@@ -282,55 +288,30 @@ __device__ __forceinline__ void relax(const Phys<C> &P, const C x[9], C o[9])
     vy = rf * vy;
     vxpy = rf * vxpy;
     vxmy = rf * vxmy;
-    const C rhoe = SCALED ? rho * P.usc : rho;
-    C rp0 = rhoe * p0, rp1 = rhoe * p1, rp2 = rhoe * p2;
-    // the density term cte1*(rp - p) of the reference, or p*drho
-    const C d0 = form ? p0 * drs : cte1 * (rp0 - p0);
-    const C d1 = form ? p1 * drs : cte1 * (rp1 - p1);
-    const C d2 = form ? p2 * drs : cte1 * (rp2 - p2);
-    C e01 = rp2 * (+vxmy + qxmy) + d2;
-    C e03 = rp2 * (+vxpy + qxpy) + d2;
-    C e05 = rp1 * (+vx + qx) + d1;
-    C e08 = rp1 * (+vy + qy) + d1;
-    C e10 = rp2 * (-vxpy + qxpy) + d2;
-    C e12 = rp2 * (-vxmy + qxmy) + d2;
-    C e14 = rp1 * (-vx + qx) + d1;
-    C e17 = rp1 * (-vy + qy) + d1;
-    C e19 = rp0 * (+q0) + d0;
-    C om = P.omega;
-    if (P.les) {
-        // Smagorinsky LES (SRC/col_MC.F90:178-230, SRC/col.F90:154-205):
-        // the relaxation rate from the non-equilibrium stress; omega1 is the
-        // molecular rate of hencol (= P.omega)
-        const C n01 = x01 - e01, n03 = x03 - e03, n05 = x05 - e05, n08 = x08 - e08;
-        const C n10 = x10 - e10, n12 = x12 - e12, n14 = x14 - e14, n17 = x17 - e17;
-        const C pxx = n01 + n03 + n05 + n10 + n12 + n14;
-        const C pyy = n01 + n03 + n08 + n10 + n12 + n17;
-        const C pxy = -n01 + n03 + n10 - n12;
-        const C pyx = pxy;
-        C ptot = sqrt(pxx * pxx + (C)2.0 * pxy * pyx + pyy * pyy);
-        if (SCALED) ptot = ptot * P.dsc;
-        // Ts = 1/(2 omega1) + sqrt(18 cteS^2 P + (1/omega1)^2) / 2
-        const C ts = P.les_h + sqrt(P.les_c * ptot + P.les_i2) / (C)2.0;
-        om = (C)1.0 / ts;
-    }
+    C rp0 = rho * p0, rp1 = rho * p1, rp2 = rho * p2;
+    const C d0 = cte1 * (rp0 - p0), d1 = cte1 * (rp1 - p1), d2 = cte1 * (rp2 - p2);
+    C e[9];
+    e[0] = rp2 * (+vxmy + qxmy) + d2;
+    e[1] = rp2 * (+vxpy + qxpy) + d2;
+    e[2] = rp1 * (+vx + qx) + d1;
+    e[3] = rp1 * (+vy + qy) + d1;
+    e[4] = rp2 * (-vxpy + qxpy) + d2;
+    e[5] = rp2 * (-vxmy + qxmy) + d2;
+    e[6] = rp1 * (-vx + qx) + d1;
+    e[7] = rp1 * (-vy + qy) + d1;
+    e[8] = rp0 * (+q0) + d0;
+    const C om = P.les ? les_rate<C, false>(P, x, e) : P.omega;
     C t[9];
-    t[0] = x01 - om * (x01 - e01);
-    t[1] = x03 - om * (x03 - e03);
-    t[2] = x05 - om * (x05 - e05);
-    t[3] = x08 - om * (x08 - e08);
-    t[4] = x10 - om * (x10 - e10);
-    t[5] = x12 - om * (x12 - e12);
-    t[6] = x14 - om * (x14 - e14);
-    t[7] = x17 - om * (x17 - e17);
-    o[8] = x19 - om * (x19 - e19);
+#pragma unroll
+    for (int k = 0; k < 9; k++) t[k] = x[k] - om * (x[k] - e[k]);
+    o[8] = t[8];
     const bool fon = FORCE && P.channel_force != 0;
-    if (fon || (form == 0 && !Z0)) {
+    if (fon || !Z0) {
         // forcing term (SRC/col_MC.F90:266-274, SRC/col.F90:210-218):
         // fgrad*rho on x, or on y in the FORCING_Y build
         C forcex = (C)0.0, forcey = (C)0.0;
-        if (FORCE && P.channel_force == 1) forcex = P.fgrad * rhoe;
-        if (FORCE && P.channel_force == 2) forcey = P.fgrad * rhoe;
+        if (FORCE && P.channel_force == 1) forcex = P.fgrad * rho;
+        if (FORCE && P.channel_force == 2) forcey = P.fgrad * rho;
         o[0] = t[0] + (+forcex - forcey);
         o[1] = t[1] + (+forcex + forcey);
         o[2] = t[2] + (+forcex);
@@ -343,6 +324,82 @@ __device__ __forceinline__ void relax(const Phys<C> &P, const C x[9], C o[9])
 #pragma unroll
         for (int k = 0; k < 8; k++) o[k] = t[k];
     }
+}
+
+// Form 1, the deviation form (an extension of the reference): the density
+// deviation drho = sum of the shifted populations is never added to 1 where
+// it multiplies a weight (the w*drho terms), so the O(u^2) deviation is not
+// rounded against 1, and every multiply-add is a fused one:
+//   u = j / rho, usq = ux^2 + uy^2, m = -1.5 usq,
+//   a_w = fma(w rho, m, w drho), b_w = 4.5 w rho, c_w = 3 w rho,
+//   e_k, e_k' = fma(b_w, cu^2, a_w) +- c_w cu for each opposite pair,
+//   e_19 = a_0,   o_k = fma(omega, e_k - x_k, x_k),
+// plus the CHANNEL body force.  SCALED: x and o are the populations as
+// stored, scale*(f - w) (P.usc = scale, P.dsc = 1/scale); scaling by a
+// power of two commutes with every rounding here, so each result is
+// exactly scale times the unscaled one and the store needs no multiply.
+template <class C, bool FORCE, bool SCALED>
+__device__ __forceinline__ void relax_dev(const Phys<C> &P, const C x[9], C o[9])
+{
+    const C p0 = K<C>::p0, p1 = K<C>::p1, p2 = K<C>::p2;
+    const C x01 = x[0], x03 = x[1], x05 = x[2], x08 = x[3], x10 = x[4];
+    const C x12 = x[5], x14 = x[6], x17 = x[7], x19 = x[8];
+    const C drs = ((x01 + x03 + x10 + x12) + (x05 + x08 + x14 + x17)) + x19;  // scaled when SCALED
+    const C drho = SCALED ? drs * P.dsc : drs;
+    const C rho = (C)1.0 + drho;
+    const C rhoinv = (C)1.0 / rho;
+    const C rinv = SCALED ? rhoinv * P.dsc : rhoinv;
+    const C ux = ((x03 - x12) + (x01 - x10) + (x05 - x14)) * rinv;
+    const C uy = ((x03 - x01) + (x12 - x10) + (x08 - x17)) * rinv;
+    const C ux2 = ux * ux, uy2 = uy * uy;
+    const C m = (C)(-1.5) * (ux2 + uy2);
+    const C rw = SCALED ? rho * P.usc : rho;
+    const C r0 = rw * p0, r1 = rw * p1, r2 = rw * p2;
+    const C a0 = fma(r0, m, p0 * drs), a1 = fma(r1, m, p1 * drs), a2 = fma(r2, m, p2 * drs);
+    const C b1 = (C)4.5 * r1, b2 = (C)4.5 * r2, c1 = (C)3.0 * r1, c2 = (C)3.0 * r2;
+    C e[9];
+    {
+        const C sx = fma(b1, ux2, a1), tx = c1 * ux;
+        e[2] = sx + tx;  // 05
+        e[6] = sx - tx;  // 14
+        const C sy = fma(b1, uy2, a1), ty = c1 * uy;
+        e[3] = sy + ty;  // 08
+        e[7] = sy - ty;  // 17
+        const C up = ux + uy, sp = fma(b2, up * up, a2), tp = c2 * up;
+        e[1] = sp + tp;  // 03
+        e[4] = sp - tp;  // 10
+        const C um = ux - uy, sm = fma(b2, um * um, a2), tm = c2 * um;
+        e[0] = sm + tm;  // 01
+        e[5] = sm - tm;  // 12
+        e[8] = a0;       // 19
+    }
+    const C om = P.les ? les_rate<C, SCALED>(P, x, e) : P.omega;
+#pragma unroll
+    for (int k = 0; k < 9; k++) o[k] = fma(om, e[k] - x[k], x[k]);
+    if (FORCE && P.channel_force != 0) {
+        C forcex = (C)0.0, forcey = (C)0.0;
+        if (P.channel_force == 1) forcex = P.fgrad * rw;
+        if (P.channel_force == 2) forcey = P.fgrad * rw;
+        o[0] = o[0] + (+forcex - forcey);
+        o[1] = o[1] + (+forcex + forcey);
+        o[2] = o[2] + (+forcex);
+        o[3] = o[3] + (+forcey);
+        o[4] = o[4] + (-forcex - forcey);
+        o[5] = o[5] + (-forcex + forcey);
+        o[6] = o[6] + (-forcex);
+        o[7] = o[7] + (-forcey);
+    }
+}
+
+// FL: -1 the form is P.form; 0 or 1 that form, fixed at compile time.
+template <class C, bool FORCE = true, int FL = -1, bool SCALED = false, bool Z0 = false>
+__device__ __forceinline__ void relax(const Phys<C> &P, const C x[9], C o[9])
+{
+    static_assert(!SCALED || FL == 1, "scaled populations need the deviation form");
+    if (FL == 1 || (FL < 0 && P.form))
+        relax_dev<C, FORCE, SCALED>(P, x, o);
+    else
+        relax_ref<C, FORCE, Z0>(P, x, o);
 }
 
 // Shifted equilibrium at rho = 1 with the quadratic part in double

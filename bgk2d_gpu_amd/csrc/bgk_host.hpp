@@ -391,7 +391,7 @@ int size_tb(bgk_ctx *c)
     hipError_t e = hipSuccess;
     with_fl<S>(c, [&](auto flc) {
         constexpr int FL = decltype(flc)::value;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused_tb<S, C, BC, false, FL>, 256, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused_tb<S, C, BC, false, FL>, TB_TW, 0);
     });
     if (e != hipSuccess) return fail(c, BGK_E_HIP, "hipOccupancyMaxActiveBlocksPerMultiprocessor: %s", hipGetErrorString(e));
     HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->dev));
@@ -443,14 +443,14 @@ int launch_tb_t(bgk_ctx *c)
         HIPCHK(c, hipStreamWaitEvent(c->s2, c->evf, 0));
         with_fl<S>(c, [&](auto flc) {
             constexpr int FL = decltype(flc)::value;
-            hipLaunchKernelGGL((k_fused_tb_box<S, C, BC, FL>), dim3(nbx * nsub), dim3(256), 0, c->s2, L, P, box.x,
+            hipLaunchKernelGGL((k_fused_tb_box<S, C, BC, FL>), dim3(nbx * nsub), dim3(TB_TW), 0, c->s2, L, P, box.x,
                                nbx, r0, r1, hb);
         });
         HIPCHK(c, hipEventRecord(c->evj, c->s2));
     }
     with_fl<S>(c, [&](auto flc) {
         constexpr int FL = decltype(flc)::value;
-        hipLaunchKernelGGL((k_fused_tb<S, C, BC, false, FL>), dim3(strips * rows8), dim3(256), 0, c->s, L, P, H,
+        hipLaunchKernelGGL((k_fused_tb<S, C, BC, false, FL>), dim3(strips * rows8), dim3(TB_TW), 0, c->s, L, P, H,
                            strips, rows, box);
     });
     if (OBS && box.y >= box.x) HIPCHK(c, hipStreamWaitEvent(c->s, c->evj, 0));
@@ -941,7 +941,7 @@ void launch_batch_s(bgk_batch_t *b, int g, int nb, int t, hipStream_t s)
 template <class S>
 void launch_batch_tb(bgk_batch_t *b, int g, int nb, int t2, hipStream_t s)
 {
-    const dim3 grid(nb), blk(256);
+    const dim3 grid(nb), blk(bgk::TB_TW);
     const bgk::Member *dm = b->d_members;
     const int2 *dt = b->d_tiles;
     const int *meta = b->d_ntiles;
@@ -972,12 +972,12 @@ template <class S>
 int occupancy_tb(int v, int *per_cu)
 {
     switch (bgk::BATCH_BCS[v]) {
-    case bgk::BK_PERIODIC: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_PERIODIC>, 256, 0);
-    case bgk::BK_CHANNEL: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_CHANNEL>, 256, 0);
-    case bgk::BK_INFLOW: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_INFLOW>, 256, 0);
-    case bgk::BK_COUETTE: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_COUETTE>, 256, 0);
-    case bgk::BK_OPEN: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_OPEN>, 256, 0);
-    default: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_DRIVEN>, 256, 0);
+    case bgk::BK_PERIODIC: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_PERIODIC>, bgk::TB_TW, 0);
+    case bgk::BK_CHANNEL: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_CHANNEL>, bgk::TB_TW, 0);
+    case bgk::BK_INFLOW: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_INFLOW>, bgk::TB_TW, 0);
+    case bgk::BK_COUETTE: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_COUETTE>, bgk::TB_TW, 0);
+    case bgk::BK_OPEN: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_OPEN>, bgk::TB_TW, 0);
+    default: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_DRIVEN>, bgk::TB_TW, 0);
     }
 }
 // blocks of the two-step batch kernel of storage kind `kind`, variant v,

@@ -755,7 +755,12 @@ __global__ __launch_bounds__(256) void k_fused2(Lat L, Phys<C> P)
 // Ring hazard: phase 2 of row r-1 reads ring rows r-2..r; phase 1 of row
 // r+1 overwrites ring row r-3, read last by phase 2 of row r-2, which every
 // thread has finished when it passes the barrier of row r.
-constexpr int TB_TW = 256;  // strip width (one column per thread)
+// strip width of the two-step kernels: one column per thread, one block
+// per strip
+#ifndef BGK_TB_TW
+#define BGK_TB_TW 256
+#endif
+constexpr int TB_TW = BGK_TB_TW;
 constexpr int TB_RB = 16;   // rows of rim columns computed per batch (power of two; 16 beat 8, 32, 64)
 
 // ring slot of population k (0..7, not 08/17): 01,03,05,10,12,14 -> 0..5
@@ -828,10 +833,9 @@ __device__ __forceinline__ void tb_block(const Lat &L, const Phys<C> &P, int bx,
     for (int d = 0; d < 3; d++) acc.h08[d] = acc.h17[d] = acc.h19[d] = R{};
     const int e = tid + 1, i = i0 + tid;
     const bool own = tid < ncols;
-    R xr[9];
-    int wr = wrow(j0 - 1);
-    if (own && wr >= 0) tb_load<S>(L, i, wr, xr);
-    for (int r = j0 - 1; r <= jend + 1; r++) {
+    // one row r of the march: the pulls of row r are in xa (node row wa);
+    // after phase 1 the buffer receives those of row r+1
+    auto row = [&](int r, R(&xa)[9], int &wa) {
         const int slot = (r - j0 + 1) & 3;
         if (((r - j0 + 1) & (RB - 1)) == 0) {
             // rim columns, step t+1 of the next RB rows: column i0-1 keeps
@@ -839,7 +843,7 @@ __device__ __forceinline__ void tb_block(const Lat &L, const Phys<C> &P, int bx,
             // 10,12,14.  The previous segment's rim rows were all copied
             // into the ring before the barrier of their own row.
             const int nr = min(RB, jend + 2 - r);
-            for (int t = tid; t < 2 * nr; t += 256) {
+            for (int t = tid; t < 2 * nr; t += TB_TW) {
                 const int side = t >= nr ? 1 : 0, rr = t - side * nr;
                 const int wi = wcol(side ? i0 + ncols : i0 - 1), wj = wrow(r + rr);
                 if (wi < 0 || wj < 0) continue;
@@ -863,10 +867,10 @@ __device__ __forceinline__ void tb_block(const Lat &L, const Phys<C> &P, int bx,
             acc.h17[d] = acc.h17[d + 1];
             acc.h19[d] = acc.h19[d + 1];
         }
-        if (own && wr >= 0) {
+        if (own && wa >= 0) {
             C o[9];
-            tb_node<S, C, BC, OBS, FL>(L, P, i, wr, xr, o);
-            const uint32_t salt = P.rounding ? salt_of(i, wr) : 0u;
+            tb_node<S, C, BC, OBS, FL>(L, P, i, wa, xa, o);
+            const uint32_t salt = P.rounding ? salt_of(i, wa) : 0u;
 #pragma unroll
             for (int k = 0; k < 8; k++)
                 if (cx_of(k) != 0) ring[slot][tb_slot(k)][e] = encode_s<S, C>(P, o[k], salt, k);
@@ -879,12 +883,12 @@ __device__ __forceinline__ void tb_block(const Lat &L, const Phys<C> &P, int bx,
             ring[slot][side ? k + 3 : k][side ? ncols + 1 : 0] = rim[side][(r - j0 + 1) & (RB - 1)][k];
         }
         // prefetch the raw pulls of row r+1
-        wr = r + 1 <= jend + 1 ? wrow(r + 1) : -1;
-        if (own && wr >= 0) tb_load<S>(L, i, wr, xr);
+        wa = r + 1 <= jend + 1 ? wrow(r + 1) : -1;
+        if (own && wa >= 0) tb_load<S>(L, i, wa, xa);
         __syncthreads();
         // phase 2: step t+2 of row q = r-1 (history rows r-2, r-1, r)
         const int q = r - 1;
-        if (q < j0 || !own) continue;
+        if (q < j0 || !own) return;
         acc.q = q;
         C x[9], o[9];
 #pragma unroll
@@ -900,7 +904,11 @@ __device__ __forceinline__ void tb_block(const Lat &L, const Phys<C> &P, int bx,
 #pragma unroll
         for (int k = 0; k < 8; k++) st_nt((R *)(pull_base<S>(L, L.b[k], 8) + bo), encode_s<S, C>(P, o[k], salt, k));
         st_nt((R *)(pull_base<S>(L, L.rn, 8) + bo), encode_s<S, C>(P, o[8], salt, 8));
-    }
+    };
+    R xa[9];
+    int wa = wrow(j0 - 1);
+    if (own && wa >= 0) tb_load<S>(L, i, wa, xa);
+    for (int r = j0 - 1; r <= jend + 1; r++) row(r, xa, wa);
 }
 
 // Blocks are dispatched to the eight XCDs (each with its own L2) in turn:
@@ -921,7 +929,7 @@ __device__ __forceinline__ bool tb_xcd_block(int b, int strips, int rows, int &b
 // box = (first, last strip; first, last row of strip blocks): blocks left
 // to k_fused_tb_box (an empty range, x > y, skips none)
 template <class S, class C, int BC, bool OBS, int FL = -1>
-__global__ __launch_bounds__(256) void k_fused_tb(Lat L, Phys<C> P, int H, int strips, int rows, int4 box)
+__global__ __launch_bounds__(TB_TW) void k_fused_tb(Lat L, Phys<C> P, int H, int strips, int rows, int4 box)
 {
     int bx, by;
     if (!tb_xcd_block((int)blockIdx.x, strips, rows, bx, by)) return;
@@ -936,7 +944,7 @@ __global__ __launch_bounds__(256) void k_fused_tb(Lat L, Phys<C> P, int H, int s
 // carry it, shorter than the others so that they end with them, and the
 // grid of the others keeps its occupancy.
 template <class S, class C, int BC, int FL = -1>
-__global__ __launch_bounds__(256) void k_fused_tb_box(Lat L, Phys<C> P, int bx0, int nbx, int r0, int r1, int hb)
+__global__ __launch_bounds__(TB_TW) void k_fused_tb_box(Lat L, Phys<C> P, int bx0, int nbx, int r0, int r1, int hb)
 {
     const int j0 = r0 + ((int)blockIdx.x / nbx) * hb;
     tb_block<S, C, BC, true, FL>(L, fixed_flags<S, C, BC, FL>(P), bx0 + (int)blockIdx.x % nbx, j0,

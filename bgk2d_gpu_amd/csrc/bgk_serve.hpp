@@ -179,7 +179,7 @@ __device__ __forceinline__ Phys<float> cld_phys(const Phys<float> *p)
 }
 
 template <class S, int BC>
-__global__ __launch_bounds__(256) void k_batch_tb(const Member *__restrict__ M, const int2 *__restrict__ tasks,
+__global__ __launch_bounds__(TB_TW) void k_batch_tb(const Member *__restrict__ M, const int2 *__restrict__ tasks,
                                                   const int *__restrict__ meta, int group, int t2)
 {
     const int n = cld(meta + group);
