@@ -67,12 +67,15 @@ __host__ __device__ constexpr uint32_t sr_mul(int k)
     return k == 0 ? 0xB5297Bu : k == 1 ? 0x9E3779u : k == 2 ? 0xC2B2AFu : k == 3 ? 0x85EBCBu : k == 4 ? 0xD35A2Du
          : k == 5 ? 0xA3B195u : k == 6 ? 0xE6546Bu : k == 7 ? 0x8B1E5Fu : 0xF1357Bu;
 }
-// per-store random word: one 24-bit multiply-add (v_mad_u32_u24) of the low
-// bits of the stored value, which change from step to step, and the node's
-// salt; the conversion takes its top bits
+// per-store random word: the low bits of the stored value, which change
+// from step to step, mixed with the node's salt, times population k's
+// multiplier (a xor and a full-rate 24-bit multiply); the conversion takes
+// its top bits.  Mixing the salt in before the multiply makes the word a
+// non-linear function of the dropped bits (adding it after the multiply
+// left the word linear in them and the BF16 error 3.6 times larger).
 __host__ __device__ inline uint32_t sr_word(uint32_t bits, uint32_t salt, int k)
 {
-    return mul24(bits, sr_mul(k)) + salt;
+    return mul24(bits ^ salt, sr_mul(k));
 }
 
 typedef __fp16 sr_half2 __attribute__((ext_vector_type(2)));

@@ -93,14 +93,17 @@ typedef struct bgk_params {
                              reference build does (SRC/hencol.F90:33-46)    */
     int32_t form;         /* 0: the reference's operation order (bit-exact
                              with its builds); 1: deviation form, the
-                             density deviation kept apart from 1           */
+                             density deviation kept apart from 1 and fused
+                             multiply-adds (an extension)                  */
     int32_t rounding;     /* FP16/BF16 stores: 0 round to nearest even (the
                              reference's MIXED1); 1 deterministic stochastic
-                             rounding (hash of value and position)          */
+                             rounding (random word from the value bits and
+                             the position)                                 */
     int32_t tblock;       /* FUSED steps per pass over HBM in bgk_run: 0 the
-                             library's choice (2 for fp32/fp64 storage, 1
-                             for FP16/BF16), 1 one step per kernel, 2 two
-                             steps per kernel; results are identical       */
+                             library's choice (2 where the lattice fills the
+                             device with strips of >= 8 rows, else 1), 1 one
+                             step per kernel, 2 two steps per kernel;
+                             results are identical                         */
     int32_t les;          /* 1: Smagorinsky LES collision, the -DLES build
                              (SRC/col_MC.F90:178-230)                       */
     int32_t trick1;       /* 1: driven-cavity corner order of the TRICK1 build
@@ -111,7 +114,7 @@ typedef struct bgk_params {
                              without implicit padding                       */
     double svisc;         /* namelist svisc (kinematic viscosity)           */
     double u0;            /* namelist u0 (lid speed, or channel force scale)*/
-    double scale;         /* FP16/BF16 storage scale (power of two, >= 1)   */
+    double scale;         /* FP16/BF16 storage scale (power of two)         */
     double cte_s;         /* LES: namelist cteS (SRC/input.F90:55 defaults it
                              to 0.1, a default-real literal)                */
 } bgk_params_t;

@@ -45,3 +45,27 @@ def test_fp16_error_vs_double():
     # stochastic rounding with the deviation form meets the 1e-3 bar
     assert errs[("fp16", 1, 1)] <= 1e-3
     assert errs[("fp16", 1, 1)] < errs[("fp16", 0, 0)]
+
+
+def test_c3_full_size_fp16_error_vs_double():
+    """The precision configuration at its full size: Taylor-Green 8192^2,
+    tau = 0.8 (svisc = 0.1), u0 = 0.01, 5000 steps.  FP16 storage (scale
+    1024, deviation form, stochastic rounding) against the FP64 run:
+    velocity relative L2 over all nodes and both components <= 1e-3
+    (u, v per node from bgk_fields, FP32 quotients of the population sums)."""
+    n, steps = 8192, 5000
+
+    def uv(prec, scale, form, rnd):
+        s = Solver(n, n, bc="periodic", prec=prec, svisc=0.1, amplitude=0.01, scale=scale,
+                   form=form, rounding=rnd)
+        s.run(steps)
+        u, v, _ = s.fields()
+        s.close()
+        return u.astype(np.float64), v.astype(np.float64)
+
+    ur, vr = uv("double", 1.0, 0, 0)
+    norm = np.sqrt((ur ** 2 + vr ** 2).sum())
+    u, v = uv("fp16", 1024.0, 1, 1)
+    err = np.sqrt(((u - ur) ** 2 + (v - vr) ** 2).sum()) / norm
+    print(f"C3 8192^2 x 5000: FP16 relative L2 vs FP64 = {err:.3e}", flush=True)
+    assert err <= 1e-3
