@@ -151,12 +151,34 @@ int bgk_build_obs(bgk_ctx_t *c)
     int r = check_ctx(c);
     if (r) return r;
     // SRC/build.obs.F90:49-205, centred at (2l/5, m/2), comparisons in
-    // mykind: obstacle 1 (and 0) the cylinder (icoord-i)^2+(jcoord-j)^2 <=
+    // mykind: obstacle 4 the porous layout (below), 1 (and 0) the cylinder
+    // (icoord-i)^2+(jcoord-j)^2 <=
     // R*R (default branch), 2 the square of half side radius (SQUARE), 3 the
     // cylinder's rows m/2..m (HALF_CIRCLE)
     const int l = c->l, m = c->m, ic = 2 * l / 5, jc = m / 2, shape = c->p.obstacle;
     std::vector<int32_t> obs((size_t)l * m, 0);
     const bool dbl = c->p.prec == BGK_PREC_DOUBLE || c->p.prec == BGK_PREC_MIXED2;
+    if (shape == 4) {
+        // POROUS (SRC/build.obs.F90:117-147): a plus of five solid nodes
+        // centred at (i, j), j = 5, 10, .. <= m-4, i = l/4, l/4+5, .. <=
+        // 3l/4, wherever the next uniform draw exceeds 0.4.  The reference
+        // draws from its compiler's RAND extension; here the draws come
+        // from the Park-Miller minimal standard generator (x <- 16807 x mod
+        // 2^31-1, seed 1, draw x / (2^31-1), the generator gfortran
+        // documents for RAND), compared in the compute kind.
+        if (l < 12 || m < 10) return fail(c, BGK_E_INVALID, "POROUS needs l >= 12, m >= 10");
+        uint64_t x = 1;
+        for (int j = 5; j <= m - 4; j += 5)
+            for (int i = l / 4; i <= 3 * l / 4; i += 5) {
+                x = x * 16807u % 2147483647u;
+                const bool hit = dbl ? (double)x / 2147483647.0 > (double)0.4f
+                                     : (float)((double)x / 2147483647.0) > 0.4f;
+                if (!hit) continue;
+                const int di[5] = {1, 0, 0, -1, 0}, dj[5] = {0, 1, 0, 0, -1};
+                for (int q = 0; q < 5; q++) obs[(size_t)(i + di[q] - 1) + (size_t)(j + dj[q] - 1) * l] = 1;
+            }
+        return set_mask(c, obs.data());
+    }
     for (int j = 1; j <= m; j++)
         for (int i = 1; i <= l; i++) {
             long long d2i = (long long)(ic - i) * (ic - i) + (long long)(jc - j) * (jc - j);
@@ -495,9 +517,9 @@ int bgk_batch_add(bgk_batch_t *b, const bgk_params_t *p, int32_t init_v, double 
 {
     if (!b || !p || !slot || nsteps < 0) return BGK_E_INVALID;
     *slot = -1;
-    if (p->impl != BGK_IMPL_FUSED || p->bc == BGK_BC_STEP || p->obstacle ||
+    if (p->impl != BGK_IMPL_FUSED || p->bc == BGK_BC_STEP || p->obstacle || p->noshift ||
         !(p->prec == BGK_PREC_SINGLE || p->prec == BGK_PREC_FP16 || p->prec == BGK_PREC_BF16))
-        return bfail(b, BGK_E_INVALID, "batch members are FUSED, without obstacle or step, fp32/fp16/bf16");
+        return bfail(b, BGK_E_INVALID, "batch members are FUSED, shifted, without obstacle or step, fp32/fp16/bf16");
     if (p->device != b->dev) return bfail(b, BGK_E_INVALID, "member on device %d, batch on %d", p->device, b->dev);
     if (const char *why = param_error(p)) return bfail(b, BGK_E_INVALID, "%s", why);
     int tw_log2, th;

@@ -222,6 +222,7 @@ struct Phys {
     // kernels that compute on scaled populations (scale*(f - w) as stored,
     // see fixed_flags): the storage scale and its inverse; 1 otherwise
     C usc, dsc;
+    int noshift;  // 1: whole populations f (the NOSHIFT build), per-phase kernels only
 };
 
 // ---------------------------------------------------------- arithmetic
@@ -268,10 +269,12 @@ __device__ __forceinline__ C les_rate(const Phys<C> &P, const C x[9], const C e[
 // The cte0 + q terms of the four non-rest equilibria are left out too: q is
 // qf*(3a*a - vsq) with 3a*a >= +0, never -0, so 0 + q = q.  The rest
 // population's q0 = cte0 + qf*(-vsq) keeps it (-vsq is -0 at rest).
-template <class C, bool FORCE, bool Z0>
+// NS: the kernel may serve the NOSHIFT build (P.noshift): cte1 = 0 and
+// cte0 = 1 (SRC/col_MC.F90:70-74), so the cte0 terms are added.
+template <class C, bool FORCE, bool Z0, bool NS = false>
 __device__ __forceinline__ void relax_ref(const Phys<C> &P, const C x[9], C o[9])
 {
-    const C cte1 = (C)1.0, cte0 = (C)1.0 - cte1;
+    const C cte1 = (NS && P.noshift) ? (C)0.0 : (C)1.0, cte0 = (C)1.0 - cte1;
     const C rf = K<C>::rf, qf = K<C>::qf, p0 = K<C>::p0, p1 = K<C>::p1, p2 = K<C>::p2;
     const C x01 = x[0], x03 = x[1], x05 = x[2], x08 = x[3], x10 = x[4];
     const C x12 = x[5], x14 = x[6], x17 = x[7], x19 = x[8];
@@ -286,6 +289,12 @@ __device__ __forceinline__ void relax_ref(const Phys<C> &P, const C x[9], C o[9]
     C qxmy = qf * ((C)3.0 * vxmy * vxmy - vsq);
     C qx = qf * ((C)3.0 * vx2 - vsq);
     C qy = qf * ((C)3.0 * vy2 - vsq);
+    if (NS) {
+        qxpy = cte0 + qxpy;
+        qxmy = cte0 + qxmy;
+        qx = cte0 + qx;
+        qy = cte0 + qy;
+    }
     C q0 = cte0 + qf * (-vsq);
     vx = rf * vx;
     vy = rf * vy;
@@ -395,14 +404,15 @@ __device__ __forceinline__ void relax_dev(const Phys<C> &P, const C x[9], C o[9]
 }
 
 // FL: -1 the form is P.form; 0 or 1 that form, fixed at compile time.
-template <class C, bool FORCE = true, int FL = -1, bool SCALED = false, bool Z0 = false>
+// NS: see relax_ref (the per-phase kernels).
+template <class C, bool FORCE = true, int FL = -1, bool SCALED = false, bool Z0 = false, bool NS = false>
 __device__ __forceinline__ void relax(const Phys<C> &P, const C x[9], C o[9])
 {
     static_assert(!SCALED || FL == 1, "scaled populations need the deviation form");
     if (FL == 1 || (FL < 0 && P.form))
         relax_dev<C, FORCE, SCALED>(P, x, o);
     else
-        relax_ref<C, FORCE, Z0>(P, x, o);
+        relax_ref<C, FORCE, Z0, NS>(P, x, o);
 }
 
 // Shifted equilibrium at rho = 1 with the quadratic part in double

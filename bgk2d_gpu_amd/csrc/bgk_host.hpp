@@ -150,6 +150,7 @@ Phys<C> phys(const bgk_ctx *c)
     P.les_c = (C)18.0 * (cs * cs);
     P.usc = (C)1.0;
     P.dsc = (C)1.0;
+    P.noshift = c->p.noshift;
     return P;
 }
 
@@ -290,7 +291,12 @@ int launch_collision(bgk_ctx *c)
     return BGK_OK;
 }
 
-inline bool foldable(const bgk_ctx *c) { return c->p.impl == BGK_IMPL_FUSED && c->p.bc != BGK_BC_STEP; }
+// the boundary folded into the hot kernels: FUSED, all variants but STEP,
+// on deviations f - w (NOSHIFT takes the per-phase kernels)
+inline bool foldable(const bgk_ctx *c)
+{
+    return c->p.impl == BGK_IMPL_FUSED && c->p.bc != BGK_BC_STEP && !c->p.noshift;
+}
 
 template <class S, class C, int BC, bool OBS>
 void launch_fused_t(bgk_ctx *c)
@@ -778,7 +784,9 @@ inline const char *param_error(const bgk_params_t *p)
         return "trick1 needs the square driven cavity";
     // FORCING_Y changes the CHANNEL build's forcing term only
     if (p->forcing_y != 0 && (p->forcing_y != 1 || p->bc != BGK_BC_CHANNEL)) return "forcing_y needs the channel";
-    if (p->obstacle < 0 || p->obstacle > 3) return "obstacle must be 0..3";
+    if (p->obstacle < 0 || p->obstacle > 4) return "obstacle must be 0..4";
+    if (p->noshift != 0 && (p->noshift != 1 || p->form != 0 || p->bc == BGK_BC_OPEN))
+        return "noshift needs form 0 and a reference boundary variant";
     if (p->prec == BGK_PREC_FP16 || p->prec == BGK_PREC_BF16) {
         int ex;
         const double sc = p->scale > 0.0 ? p->scale : 1.0;

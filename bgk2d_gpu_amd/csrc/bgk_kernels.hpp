@@ -124,13 +124,15 @@ __global__ void k_init(Lat L, Phys<C> P, const C *tsx, const C *tcx, const C *ts
 #pragma unroll
         for (int k = 0; k < 9; k++) {
             const double cu = cx_of(k) * u + cy_of(k) * v;
-            const double f = w[k] * drho + w[k] * (1.0 + drho) * (3.0 * cu + 4.5 * cu * cu - 1.5 * usq);
+            const double f = w[k] * drho + w[k] * (1.0 + drho) * (3.0 * cu + 4.5 * cu * cu - 1.5 * usq) +
+                             (P.noshift ? w[k] : 0.0);
             put<S, C>(L, P, L.a, k, i, j, (C)f);
         }
         return;
     }
     const C rf = K<C>::rf, qf = K<C>::qf, p0 = K<C>::p0, p1 = K<C>::p1, p2 = K<C>::p2;
-    const C crho = (C)1.0, cte1 = (C)0.0;
+    // NOSHIFT stores whole populations (SRC/init.F90:73-77)
+    const C crho = (C)1.0, cte1 = P.noshift ? (C)1.0 : (C)0.0;
     C cvsq = xj * xj + yj * yj;
     C c01 = eq_d<C>(xj - yj, cvsq);
     C c03 = eq_d<C>(xj + yj, cvsq);
@@ -209,7 +211,7 @@ __device__ void outflow_col(const Lat &L, const Phys<C> &P, int j)
 {
     const int l = L.l, l1 = l + 1;
     const C rf = K<C>::rf, qf = K<C>::qf, p1 = K<C>::p1, p2 = K<C>::p2;
-    const C rhoinv = (C)1.0, crho = (C)1.0, cte1 = (C)0.0;
+    const C rhoinv = (C)1.0, crho = (C)1.0, cte1 = P.noshift ? (C)1.0 : (C)0.0;
     C x01 = get<S, C>(L, P, L.a, 0, l, j), x03 = get<S, C>(L, P, L.a, 1, l, j);
     C x05 = get<S, C>(L, P, L.a, 2, l, j), x08 = get<S, C>(L, P, L.a, 3, l, j);
     C x10 = get<S, C>(L, P, L.a, 4, l, j), x12 = get<S, C>(L, P, L.a, 5, l, j);
@@ -229,7 +231,7 @@ template <class S, class C>
 __device__ void inflow_col(const Lat &L, const Phys<C> &P, int j, C xj)
 {
     const C p1 = K<C>::p1, p2 = K<C>::p2;
-    const C crho = (C)1.0, cte1 = (C)0.0, yj = (C)0.0;
+    const C crho = (C)1.0, cte1 = P.noshift ? (C)1.0 : (C)0.0, yj = (C)0.0;
     C cvsq = xj * xj + yj * yj;
     C c01 = eq_d<C>(xj - yj, cvsq), c03 = eq_d<C>(xj + yj, cvsq), c05 = eq_d<C>(xj, cvsq);
     put<S, C>(L, P, L.a, 0, 0, j, crho * p2 * (cte1 + c01));
@@ -392,7 +394,7 @@ __global__ void k_col(Lat L, Phys<C> P)
 #pragma unroll
     for (int k = 0; k < 8; k++) x[k] = get<S, C>(L, P, L.b, k, i, j);
     x[8] = get<S, C>(L, P, L.a, 8, i, j);
-    relax<C>(P, x, o);
+    relax<C, true, -1, false, false, true>(P, x, o);
 #pragma unroll
     for (int k = 0; k < 9; k++) put_nt<S, C>(L, P, L.a, k, i, j, o[k]);
 }
@@ -407,7 +409,7 @@ __global__ void k_colmc(Lat L, Phys<C> P)
 #pragma unroll
     for (int k = 0; k < 8; k++) x[k] = get<S, C>(L, P, L.a, k, i - cx_of(k), j - cy_of(k));
     x[8] = get<S, C>(L, P, L.a, 8, i, j);
-    relax<C>(P, x, o);
+    relax<C, true, -1, false, false, true>(P, x, o);
 #pragma unroll
     for (int k = 0; k < 8; k++) put_nt<S, C>(L, P, L.b, k, i, j, o[k]);
     put_nt<S, C>(L, P, L.a, 8, i, j, o[8]);
@@ -964,7 +966,7 @@ template <class S, class C>
 __global__ __launch_bounds__(ROWS_T) void k_rows(Lat L, Phys<C> P, double *out)
 {
     const int j = blockIdx.x + 1;
-    const C cte1 = (C)1.0;
+    const C cte1 = P.noshift ? (C)0.0 : (C)1.0;  // NOSHIFT: rho = sum
     double acc[ROW_VALS] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = threadIdx.x + 1; i <= L.l; i += ROWS_T) {
         C a01 = get<S, C>(L, P, L.a, 0, i, j), a03 = get<S, C>(L, P, L.a, 1, i, j);
@@ -1057,7 +1059,7 @@ __global__ void k_line(Lat L, Phys<C> P, int mode, int c0, int c1, double *out)
     if (mode == 0) { i = t + 1; j = c0; }
     else if (mode == 1) { i = c0; j = t + 1; }
     else { i = c0; j = c1; }
-    const C cte1 = (C)1.0;
+    const C cte1 = P.noshift ? (C)0.0 : (C)1.0;  // NOSHIFT: rho = sum
     C a01 = get<S, C>(L, P, L.a, 0, i, j), a03 = get<S, C>(L, P, L.a, 1, i, j);
     C a05 = get<S, C>(L, P, L.a, 2, i, j), a08 = get<S, C>(L, P, L.a, 3, i, j);
     C a10 = get<S, C>(L, P, L.a, 4, i, j), a12 = get<S, C>(L, P, L.a, 5, i, j);
@@ -1142,7 +1144,8 @@ __global__ void k_fields(Lat L, Phys<C> P, float *u, float *v, float *rho)
     for (int k = 0; k < 9; k++) a[k] = (T)get<S, C>(L, P, L.a, k, i, j);
     const float us = (float)(a[0] + a[1] + a[2] - a[4] - a[5] - a[6]);
     const float ws = (float)(a[1] + a[3] + a[5] - a[0] - a[4] - a[7]);
-    const float den = (float)((a[0] + a[1] + a[2] + a[3] + a[4] + a[5] + a[6] + a[7] + a[8]) + (T)1.0);
+    const float den = (float)((a[0] + a[1] + a[2] + a[3] + a[4] + a[5] + a[6] + a[7] + a[8]) +
+                              (P.noshift ? (T)0.0 : (T)1.0));
     const size_t q = (size_t)(i - 1) + (size_t)(j - 1) * L.l;
     u[q] = us / den;
     v[q] = ws / den;

@@ -154,7 +154,7 @@ __device__ __forceinline__ T cld(const T *p)
 }
 __device__ __forceinline__ Phys<float> cld_phys(const Phys<float> *p)
 {
-    static_assert(sizeof(Phys<float>) == 20 * 4, "cld_phys copies every field of Phys");
+    static_assert(sizeof(Phys<float>) == 21 * 4, "cld_phys copies every field of Phys");
     Phys<float> r;
     r.omega = cld(&p->omega);
     r.fgrad = cld(&p->fgrad);
@@ -175,6 +175,7 @@ __device__ __forceinline__ Phys<float> cld_phys(const Phys<float> *p)
     r.les_c = cld(&p->les_c);
     r.usc = cld(&p->usc);
     r.dsc = cld(&p->dsc);
+    r.noshift = cld(&p->noshift);
     return r;
 }
 

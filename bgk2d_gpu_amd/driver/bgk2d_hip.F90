@@ -32,7 +32,11 @@
 !     trick1         1: driven-cavity corner order of the TRICK1 build
 !     forcing_y      1: POF body force along y, as the FORCING_Y build does
 !     shape          obstacle of VKS/CYL: 'CYLINDER' (default), 'SQUARE',
-!                    'HALF_CIRCLE' (the SQUARE / HALF_CIRCLE builds)
+!                    'HALF_CIRCLE', 'POROUS' (the SQUARE / HALF_CIRCLE /
+!                    POROUS builds; POROUS draws from the Park-Miller
+!                    minimal standard generator, see include/bgk2d.h)
+!     noshift        1: whole populations f stored instead of f - w, as
+!                    the NOSHIFT build does (form 0 only)
 program bgk2d_hip
   use iso_c_binding
   use bgk_hip
@@ -43,10 +47,10 @@ program bgk2d_hip
   real(8) :: svisc, u0, cteS
   character(len=16) :: case_name, precision, implementation
   real(8) :: scale, amplitude
-  integer :: phases, device, form, rounding, obs_x, obs_y, drag, les, trick1, forcing_y
+  integer :: phases, device, form, rounding, obs_x, obs_y, drag, les, trick1, forcing_y, noshift
   character(len=16) :: shape
   namelist /gpu/ case_name, precision, implementation, scale, amplitude, phases, device, &
-       form, rounding, obs_x, obs_y, drag, les, trick1, forcing_y, shape
+       form, rounding, obs_x, obs_y, drag, les, trick1, forcing_y, shape, noshift
 
   type(bgk_params_t) :: p
   type(c_ptr) :: ctx
@@ -80,6 +84,7 @@ program bgk2d_hip
   les = 0
   trick1 = 0
   forcing_y = 0
+  noshift = 0
   shape = 'CYLINDER' 
   open(15, file='bgk.input', status='old', iostat=ios)
   if (ios /= 0) error stop 'bgk.input not found'
@@ -103,7 +108,7 @@ program bgk2d_hip
   p%les = les
   p%trick1 = trick1
   p%forcing_y = forcing_y
-  p%pad0 = 0
+  p%noshift = noshift
   p%obstacle = 0
   p%scale = scale
   select case (trim(case_name))
@@ -151,8 +156,8 @@ program bgk2d_hip
        trim(implementation)
 
   ! -------------------------------------------------------- initialize
-  if (p%obstacle == 1) then
-     if (obs_x > 0 .and. obs_y > 0) then
+  if (p%obstacle >= 1) then
+     if (p%obstacle == 1 .and. obs_x > 0 .and. obs_y > 0) then
         call set_cylinder()
      else
         call bgk_ok(ctx, bgk_build_obs(ctx), 'bgk_build_obs')
@@ -286,7 +291,7 @@ contains
        call prof_j_out(it, l/2)
        rc = bgk_check(ctx, 0.5d0)
        if (rc /= 0) write(16,*) 'WARNING: non-finite or |u| > 0.5 at step ', it
-       if (drag == 1 .and. p%obstacle == 1) call draglift_out(it)
+       if (drag == 1 .and. p%obstacle >= 1) call draglift_out(it)
     end if
     if (mod(it, itsave) == 0) call save_raw(it)
   end subroutine diagnostic

@@ -19,7 +19,7 @@ module bgk_hip
   type, bind(c) :: bgk_params_t
      integer(c_int32_t) :: lx, ly, ipad, jpad, bc, obstacle, impl, prec
      integer(c_int32_t) :: radius, device, mixed1_quirk, form, rounding, tblock
-     integer(c_int32_t) :: les, trick1, forcing_y, pad0
+     integer(c_int32_t) :: les, trick1, forcing_y, noshift
      real(c_double)     :: svisc, u0, scale, cte_s
   end type bgk_params_t
 

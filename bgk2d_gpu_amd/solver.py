@@ -39,7 +39,7 @@ class BgkParams(C.Structure):
                 ("mixed1_quirk", C.c_int32), ("form", C.c_int32),
                 ("rounding", C.c_int32), ("tblock", C.c_int32),
                 ("les", C.c_int32), ("trick1", C.c_int32), ("forcing_y", C.c_int32),
-                ("pad0", C.c_int32),
+                ("noshift", C.c_int32),
                 ("svisc", C.c_double), ("u0", C.c_double), ("scale", C.c_double),
                 ("cte_s", C.c_double)]
 
@@ -110,7 +110,7 @@ class Solver:
     def __init__(self, lx, ly, bc="driven", impl="fused", prec="single", svisc=0.1, u0=0.1,
                  obstacle=0, radius=16, scale=1.0, device=0, mixed1_quirk=0, init=True,
                  init_v=0, amplitude=0.1, form=0, rounding=0, tblock=0, les=0, cte_s=0.1,
-                 trick1=0, forcing_y=0):
+                 trick1=0, forcing_y=0, noshift=0):
         self.L = load()
         self.lx, self.ly = int(lx), int(ly)
         self.prec = prec
@@ -120,7 +120,7 @@ class Solver:
                            impl=IMPL[impl], prec=PREC[prec], radius=int(radius),
                            device=int(device), mixed1_quirk=int(mixed1_quirk), form=int(form),
                            rounding=int(rounding), tblock=int(tblock), les=int(les),
-                           trick1=int(trick1), forcing_y=int(forcing_y),
+                           trick1=int(trick1), forcing_y=int(forcing_y), noshift=int(noshift),
                            svisc=float(svisc), u0=float(u0), scale=float(scale),
                            cte_s=float(cte_s))
         h = C.c_void_p()

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define BGK_ABI_VERSION 3
+#define BGK_ABI_VERSION 4
 
 /* boundary variant: the compile-time -DLDC/-DPERIODIC/-DCHANNEL/-DINFLOW/
  * -DCOUETTE/-DSTEP choice of SRC/boundaries.F90:31-56, made at run time  */
@@ -83,8 +83,11 @@ typedef struct bgk_params {
     int32_t bc;           /* enum bgk_bc                                    */
     int32_t obstacle;     /* bcond_obs with the build_obs mask: 1 cylinder
                              (VKS build), 2 square (SQUARE), 3 half
-                             cylinder (HALF_CIRCLE); a bgk_set_obstacle
-                             mask turns a 0 into 1                          */
+                             cylinder (HALF_CIRCLE), 4 porous (POROUS:
+                             five-node plus shapes on a 5-spaced lattice
+                             where a Park-Miller minimal standard draw,
+                             seed 1, exceeds 0.4); a bgk_set_obstacle mask
+                             turns a 0 into 1                               */
     int32_t impl;         /* enum bgk_impl                                  */
     int32_t prec;         /* enum bgk_prec                                  */
     int32_t radius;       /* cylinder radius (namelist radius)              */
@@ -110,8 +113,11 @@ typedef struct bgk_params {
                              (SRC/bcond.driven.F90:43-81; lx == ly)         */
     int32_t forcing_y;    /* 1: CHANNEL body force along y, the FORCING_Y
                              build (SRC/col_MC.F90:266-274)                 */
-    int32_t pad0;         /* set to 0; keeps the doubles 8-byte aligned
-                             without implicit padding                       */
+    int32_t noshift;      /* 1: whole populations f stored instead of the
+                             deviations f - w, the NOSHIFT build
+                             (SRC/col_MC.F90:70-74, SRC/init.F90:73-77);
+                             form 0 only, not OPEN; bgk_run takes the
+                             per-phase kernels                              */
     double svisc;         /* namelist svisc (kinematic viscosity)           */
     double u0;            /* namelist u0 (lid speed, or channel force scale)*/
     double scale;         /* FP16/BF16 storage scale (power of two)         */
@@ -141,8 +147,8 @@ int bgk_version(void);
 int bgk_constants(const bgk_ctx_t *ctx, double out[4]);
 
 /* Mask of SRC/build.obs.F90:49-205 for params.obstacle (cylinder, square,
- * half cylinder; centre 2l/5, m/2).  The mask must stay at least two
- * nodes from every wall (else BGK_E_INVALID).                           */
+ * half cylinder: centre 2l/5, m/2; porous: l/4..3l/4 x 5..m-4).  The mask
+ * must stay at least two nodes from every wall (else BGK_E_INVALID).    */
 int bgk_build_obs(bgk_ctx_t *ctx);
 /* Caller-supplied mask obs(1:l,1:m) as l*m int32, x fastest (1 = solid). */
 int bgk_set_obstacle(bgk_ctx_t *ctx, const int32_t *obs);

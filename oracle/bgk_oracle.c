@@ -326,6 +326,9 @@ int orc_storage_bytes(const orc *h) { return h->esize; }
 int orc_setup(orc *h)
 {
     if (h->p.init_v < 0 || h->p.init_v > 5) return -1; /* SRC/init.F90:68-71 */
+    /* NOSHIFT stores whole populations: the deviation form and the OPEN
+     * channel's inlet are defined on deviations only                     */
+    if (h->p.noshift && (h->p.form || h->p.bc == ORC_BC_OPEN)) return -1;
     h->vt->setup(h);
     return 0;
 }

@@ -51,6 +51,7 @@ typedef struct {
     int32_t les;           /* 1: Smagorinsky LES collision (-DLES build)  */
     int32_t trick1;        /* 1: driven corner order of the TRICK1 build  */
     int32_t forcing_y;     /* 1: CHANNEL force along y (FORCING_Y build)  */
+    int32_t noshift;       /* 1: whole populations f, the NOSHIFT build   */
     double svisc, u0;      /* namelist values                             */
     double amplitude;      /* TGV amplitude (reference hard-codes 0.1)    */
     double scale;          /* storage scale for half/bf16 (power of two)  */

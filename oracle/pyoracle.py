@@ -20,7 +20,12 @@ IMPL = {"original": 0, "fused": 1}
 CASE = {"LDC": ("driven", 0), "POF": ("channel", 0), "TGV": ("periodic", 0),
         "VKS": ("inflow", 1), "COU": ("couette", 0), "BFS": ("step", 0),
         "LDCLES": ("driven", 0), "VKSLES": ("inflow", 1), "LDCT1": ("driven", 0),
-        "VKSSQ": ("inflow", 2), "VKSHC": ("inflow", 3), "POFY": ("channel", 0)}
+        "VKSSQ": ("inflow", 2), "VKSHC": ("inflow", 3), "POFY": ("channel", 0),
+        "LDCNS": ("driven", 0), "TGVNS": ("periodic", 0), "POFNS": ("channel", 0),
+        "VKSNS": ("inflow", 1), "BFSNS": ("step", 0), "COUNS": ("couette", 0),
+        "VKSD": ("inflow", 1)}
+# reference build cases compiled with -DNOSHIFT
+NOSHIFT_CASES = {"LDCNS", "TGVNS", "POFNS", "VKSNS", "BFSNS", "COUNS"}
 # reference build cases compiled with -DTRICK1
 TRICK1_CASES = {"LDCT1"}
 # reference build cases compiled with -DFORCING_Y
@@ -39,6 +44,7 @@ class OrcParams(C.Structure):
                 ("mixed1_quirk", C.c_int32), ("tgv", C.c_int32),
                 ("form", C.c_int32), ("rounding", C.c_int32),
                 ("les", C.c_int32), ("trick1", C.c_int32), ("forcing_y", C.c_int32),
+                ("noshift", C.c_int32),
                 ("svisc", C.c_double), ("u0", C.c_double),
                 ("amplitude", C.c_double), ("scale", C.c_double), ("cte_s", C.c_double)]
 
@@ -104,12 +110,12 @@ class Oracle:
     def __init__(self, lx, ly, bc="driven", impl="fused", prec="single", svisc=0.1,
                  u0=0.1, obstacle=0, radius=16, amplitude=0.1, scale=1.0,
                  mixed1_quirk=0, init_v=0, tgv=None, form=0, rounding=0, les=0, cte_s=0.1,
-                 trick1=0, forcing_y=0):
+                 trick1=0, forcing_y=0, noshift=0):
         L = lib()
         if tgv is None:
             tgv = 1 if bc == "periodic" else 0
         self.p = OrcParams(lx, ly, BC[bc], obstacle, IMPL[impl], PREC[prec], radius,
-                           init_v, mixed1_quirk, tgv, form, rounding, les, trick1, forcing_y, svisc, u0, amplitude,
+                           init_v, mixed1_quirk, tgv, form, rounding, les, trick1, forcing_y, noshift, svisc, u0, amplitude,
                            scale, cte_s)
         self.prec = prec
         self.l, self.m = lx, ly
@@ -173,6 +179,14 @@ class Oracle:
                           bbox.ctypes.data_as(C.POINTER(C.c_int32)))
         return out, bbox
 
+    def obstacle(self):
+        """(mask (m, l) int32, bbox [imin, imax, jmin, jmax])"""
+        mask = np.zeros((self.m, self.l), dtype=np.int32)
+        bbox = np.zeros(4, dtype=np.int32)
+        lib().orc_get_obs(self.h, mask.ctypes.data_as(C.POINTER(C.c_int32)),
+                          bbox.ctypes.data_as(C.POINTER(C.c_int32)))
+        return mask, bbox
+
     def set_obs(self, mask):
         mask = np.ascontiguousarray(mask, dtype=np.int32)
         assert mask.shape == (self.m, self.l)
@@ -218,5 +232,7 @@ def read_ref_dump(path):
         n = (l + 2) * (m + 2)
         pops = np.frombuffer(f.read(9 * n * es), dtype=dt).reshape(9, m + 2, l + 2).copy()
         consts = np.frombuffer(f.read(24), dtype=np.float64).copy()
+        rest = f.read(4 * l * m)  # obstacle builds: the build_obs mask
+        obs = np.frombuffer(rest, dtype=np.int32).reshape(m, l).copy() if len(rest) == 4 * l * m else None
     return dict(l=l, m=m, esize=es, nsteps=nsteps, pops=pops,
-                omega=consts[0], fgrad=consts[1], u00=consts[2])
+                omega=consts[0], fgrad=consts[1], u00=consts[2], obs=obs)

@@ -15,7 +15,8 @@
 !   dump layout: int32 l, m, storage bytes, nsteps;
 !                then a01,a03,a05,a08,a10,a12,a14,a17,a19 each
 !                (0:l+1, 0:m+1), x fastest, in the storage kind;
-!                then real(8) omega, fgrad, u00.
+!                then real(8) omega, fgrad, u00;
+!                with OBSTACLE, then int32 obs(1:l,1:m), x fastest.
 ! Text diagnostics diagno/varm/prof_i/prof_j/probe of the last step land
 ! in fort.63 / fort.62 / fort.61 / fort.64 / fort.68 as in the reference.
 program ref_harness
@@ -66,6 +67,10 @@ program ref_harness
      call collision(itime)
   end do
   call system_clock(c1)
+#ifdef DRAG
+  ! drag and lift of the last step (SRC/draglift.F90), written to fort.66
+  call draglift(nsteps)
+#endif
   ! loop wall time and MLUPS as SRC/finalize.F90:60-61 defines it
   write(6,'(a,f14.6,a,f14.3)') 'HARNESS loop_s=', real(c1-c0,8)/real(crate,8), &
        ' mlups=', real(l,8)*real(m,8)*real(nsteps,8)/(real(c1-c0,8)/real(crate,8))/1.0d6
@@ -86,6 +91,10 @@ program ref_harness
   write(30) a17(0:l1,0:m1)
   write(30) a19(0:l1,0:m1)
   write(30) real(omega,8), real(fgrad,8), real(u00,8)
+#ifdef OBSTACLE
+  ! the mask build_obs made (SRC/build.obs.F90), 1 = solid
+  write(30) int(obs(1:l,1:m),4)
+#endif
   close(30)
 
   call diagno(nsteps)

@@ -68,6 +68,23 @@ CASES = [
     ("SINGLE_FUSED_POFY", 32, 12, 0.1, 0.1, 80, 16),
     ("SINGLE_ORIG_POFY", 32, 12, 0.1, 0.1, 80, 16),
     ("DOUBLE_FUSED_POFY", 32, 12, 0.1, 0.1, 80, 16),
+    # NOSHIFT: whole populations f stored instead of the deviations f - w
+    ("SINGLE_FUSED_LDCNS", 24, 20, 0.1, 0.1, 60, 16),
+    ("SINGLE_ORIG_LDCNS", 24, 20, 0.1, 0.1, 60, 16),
+    ("DOUBLE_FUSED_LDCNS", 24, 20, 0.1, 0.1, 60, 16),
+    ("SINGLE_FUSED_TGVNS", 24, 24, 0.05, 0.1, 40, 16),
+    ("DOUBLE_FUSED_TGVNS", 24, 24, 0.05, 0.1, 40, 16),
+    ("MIXED1_FUSED_TGVNS", 24, 24, 0.05, 0.1, 40, 16),
+    ("SINGLE_FUSED_POFNS", 32, 12, 0.1, 0.1, 80, 16),
+    ("SINGLE_ORIG_POFNS", 32, 12, 0.1, 0.1, 80, 16),
+    ("SINGLE_FUSED_VKSNS", 40, 24, 0.05, 0.1, 60, 4),
+    ("SINGLE_ORIG_VKSNS", 40, 24, 0.05, 0.1, 60, 4),
+    ("SINGLE_FUSED_BFSNS", 60, 20, 0.1, 0.1, 50, 16),
+    ("SINGLE_ORIG_BFSNS", 60, 20, 0.1, 0.1, 50, 16),
+    ("SINGLE_FUSED_COUNS", 24, 16, 0.1, 0.1, 60, 16),
+    # DRAG: draglift (SRC/draglift.F90) of the last step, fort.66
+    ("SINGLE_FUSED_VKSD", 100, 60, 0.02, 0.1, 400, 8),
+    ("DOUBLE_FUSED_VKSD", 100, 60, 0.02, 0.1, 400, 8),
 ]
 
 
@@ -100,6 +117,7 @@ def run_variant(name, lx, ly, svisc, u0, nsteps, radius):
         d["prof_i"] = _read_text(os.path.join(td, "fort.61"))
         d["prof_j"] = _read_text(os.path.join(td, "fort.64"))
         d["probe"] = _read_text(os.path.join(td, "fort.68"))
+        d["drag"] = _read_text(os.path.join(td, "fort.66"))
     return d
 
 
@@ -114,6 +132,10 @@ def main():
         out[key + "/consts"] = np.array([d["omega"], d["fgrad"], d["u00"]])
         for t in ("diagno", "varm", "prof_i", "prof_j", "probe"):
             out[key + "/" + t] = d[t]
+        if d["drag"].size:
+            out[key + "/drag"] = d["drag"]
+        if d["obs"] is not None:
+            out[key + "/obs"] = d["obs"]
         print(f"{name}: {d['pops'].dtype} {d['pops'].shape}")
     path = os.path.join(HERE, "ref_fixtures.npz")
     np.savez_compressed(path, **out)

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(pkg.build.LIB)
     for n in declared():
         assert hasattr(lib, n), n
-    assert lib.bgk_version() == 3
+    assert lib.bgk_version() == 4
 
 
 def test_struct_layout_matches_header():

@@ -79,7 +79,9 @@ def test_driver_rejects_unknown_case_before_gpu(tmp_path):
                                                   ("TGV", "DOUBLE", "ORIGINAL", ""),
                                                   ("VKS", "SINGLE", "FUSED", ""),
                                                   ("POF", "MIXED1", "FUSED", ""),
-                                                  ("POF", "SINGLE", "FUSED", "forcing_y=1")])
+                                                  ("POF", "SINGLE", "FUSED", "forcing_y=1"),
+                                                  ("LDC", "SINGLE", "FUSED", "noshift=1"),
+                                                  ("VKS", "DOUBLE", "ORIGINAL", "noshift=1")])
 def test_driver_matches_oracle(tmp_path, case, prec, impl, extra, phases):
     d = str(tmp_path)
     a = write_input(d, case=case, prec=prec, impl=impl, phases=phases, lx=80, ly=40,
@@ -90,7 +92,7 @@ def test_driver_matches_oracle(tmp_path, case, prec, impl, extra, phases):
     oprec = {"SINGLE": "single", "DOUBLE": "double", "MIXED1": "mixed1"}[prec]
     o = po.Oracle(80, 40, bc=bc, impl=impl.lower(), prec=oprec, svisc=0.05, u0=0.1,
                   obstacle=obst, radius=16 if not obst else 16, mixed1_quirk=int(prec == "MIXED1"),
-                  forcing_y=int(extra == "forcing_y=1"))
+                  forcing_y=int(extra == "forcing_y=1"), noshift=int(extra == "noshift=1"))
     o.step(300)
     it, pops = read_save(os.path.join(d, "save.00000300.bin"), po.STORE_DTYPE[oprec], 80, 40)
     assert it == 300

@@ -49,7 +49,8 @@ def fixture_solver(name, impl_override=None):
                radius=rad, mixed1_quirk=1 if p == "MIXED1" else 0,
                les=int(case in po.LES_CASES), cte_s=po.CTE_S_REF,
                trick1=int(case in po.TRICK1_CASES),
-               forcing_y=int(case in po.FORCING_Y_CASES))
+               forcing_y=int(case in po.FORCING_Y_CASES),
+               noshift=int(case in po.NOSHIFT_CASES))
     return s, ns
 
 
@@ -63,6 +64,34 @@ def test_reference_fixture_phases(name):
     got, ref = s.raw(), FIX[name + "/pops"]
     ok = same_bits(got, ref, s.prec)
     assert ok.all(), f"{int((~ok).sum())} differing elements"
+
+
+@pytest.mark.parametrize("name", [v for v in VARIANTS if FIX.__contains__(v + "/obs")])
+def test_reference_obstacle_mask(name):
+    """bgk_build_obs makes the mask the reference's build_obs made"""
+    s, _ = fixture_solver(name)
+    mask, _ = s.obstacle()
+    np.testing.assert_array_equal(mask, FIX[name + "/obs"])
+    s.close()
+
+
+@pytest.mark.parametrize("name", [v for v in VARIANTS if FIX.__contains__(v + "/drag")])
+def test_reference_draglift(name):
+    """bgk_draglift after the reference's steps against its draglift output
+    (SRC/draglift.F90:55-101, fort.66: force / (u_inflow^2 radius)): the
+    reference sums the per-link terms serially in its compute kind, the
+    library in FP64; SINGLE relative 2e-6, DOUBLE 1e-12 of the drag."""
+    s, ns = fixture_solver(name)
+    s.run(ns)
+    lx, ly, _, rad = (int(x) for x in FIX[name + "/params"])
+    ic, jc = 2 * lx // 5, ly // 2
+    fx, fy = s.draglift((ic - rad - 3, ic + rad + 3, jc - rad - 3, jc + rad + 3))
+    norm = 1.0 / (float(np.float32(0.1)) ** 2 * rad)
+    _, cd, cl = FIX[name + "/drag"][-1]
+    tol = 2e-6 if name.startswith("SINGLE") else 1e-12
+    assert abs(fx * norm - cd) <= tol * abs(cd)
+    assert abs(fy * norm - cl) <= tol * abs(cd)
+    s.close()
 
 
 @pytest.mark.parametrize("name", VARIANTS)

@@ -1,9 +1,11 @@
 """Pin the oracle against the reference's own code.
 
-tests/golden/ref_fixtures.npz holds, for 20 build variants (precision x
-ORIGINAL/FUSED x LDC/TGV/POF/VKS/COU/BFS), the final populations and text
-diagnostics written by the reference's hot-path Fortran sources compiled
-unchanged (oracle/Makefile, tests/golden/make_golden.py).  The oracle must
+tests/golden/ref_fixtures.npz holds, for 50 build variants (precision x
+ORIGINAL/FUSED x LDC/TGV/POF/VKS/COU/BFS, with the LES, TRICK1, SQUARE,
+HALF_CIRCLE, FORCING_Y, NOSHIFT and DRAG flags), the final populations and
+text diagnostics written by the reference's hot-path Fortran sources
+compiled unchanged (oracle/Makefile, tests/golden/make_golden.py), the
+build_obs masks of the newer obstacle variants and draglift's output.  The oracle must
 reproduce every stored population bit for bit, halos included.
 """
 import os
@@ -30,16 +32,47 @@ def run_oracle(name):
                   prec=PREC[p], svisc=sv, u0=u0, obstacle=obst, radius=rad,
                   mixed1_quirk=1 if p == "MIXED1" else 0, les=int(case in po.LES_CASES),
                   cte_s=po.CTE_S_REF, trick1=int(case in po.TRICK1_CASES),
-                  forcing_y=int(case in po.FORCING_Y_CASES))
+                  forcing_y=int(case in po.FORCING_Y_CASES), noshift=int(case in po.NOSHIFT_CASES))
     o.step(ns)
     return o
 
 
 def test_fixture_inventory():
-    assert len({v.split("@")[0] for v in VARIANTS}) == 35
+    assert len({v.split("@")[0] for v in VARIANTS}) == 50
     cases = {v.split("@")[0].split("_")[2] for v in VARIANTS}
     assert cases == {"LDC", "TGV", "POF", "VKS", "COU", "BFS", "LDCLES", "VKSLES", "LDCT1",
-                     "VKSSQ", "VKSHC", "POFY"}
+                     "VKSSQ", "VKSHC", "POFY", "LDCNS", "TGVNS", "POFNS", "VKSNS", "BFSNS",
+                     "COUNS", "VKSD"}
+
+
+@pytest.mark.parametrize("name", [v for v in VARIANTS if FIX.__contains__(v + "/obs")])
+def test_obstacle_mask_of_build_obs(name):
+    """the mask the oracle builds equals the one the reference's build_obs made"""
+    o = run_oracle(name)
+    mask, _ = o.obstacle()
+    np.testing.assert_array_equal(mask, FIX[name + "/obs"])
+
+
+@pytest.mark.parametrize("name", [v for v in VARIANTS if FIX.__contains__(v + "/drag")])
+def test_draglift_against_reference(name):
+    """draglift (SRC/draglift.F90:55-101) of the last step: the reference
+    sums 2 c_k a_k over the fluid-solid links of the box (cylinder centre
+    (2l/5, m/2) +- (radius + 3)) serially in its compute kind and prints
+    force / (u_inflow^2 radius) with u_inflow = 0.1 (a default-real
+    literal); the oracle sums the same per-link terms in long double.
+    SINGLE: relative 2e-6 (the reference's fp32 sum); DOUBLE: 1e-12.  The
+    lift is at the rounding level of the drag (symmetric flow)."""
+    o = run_oracle(name)
+    lx, ly, ns, rad = (int(x) for x in FIX[name + "/params"])
+    ic, jc = 2 * lx // 5, ly // 2
+    fx, fy = o.draglift((ic - rad - 3, ic + rad + 3, jc - rad - 3, jc + rad + 3))
+    u_inflow = float(np.float32(0.1))
+    norm = 1.0 / (u_inflow * u_inflow * rad)
+    step, cd, cl = FIX[name + "/drag"][-1]
+    assert step == ns
+    tol = 2e-6 if name.startswith("SINGLE") else 1e-12
+    assert abs(fx * norm - cd) <= tol * abs(cd)
+    assert abs(fy * norm - cl) <= tol * abs(cd)
 
 
 @pytest.mark.parametrize("name", VARIANTS)
