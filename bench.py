@@ -79,21 +79,72 @@ def fp16_leg(steps):
                 "kernel_ms_per_step_fp16": round(fp[("fp16", 2)][1]["kernel_ms"], 5)}}
 
 
-def cpu_baseline(budget_s=20.0):
-    """The reference's own FUSED SINGLE LDC sources built serially by flang
-    (oracle/_ref), on a bounded sample of the same 4096^2 workload."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "ref_SINGLE_FUSED_LDC")
+def host_info():
+    """nproc (this process's CPU set), the CPU model and the number of
+    physical cores (lscpu)"""
+    info = {"nproc": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, text=True, timeout=30).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+        cores = subprocess.run(["lscpu", "-p=Core,Socket"], stdout=subprocess.PIPE, text=True,
+                               timeout=30).stdout
+        info["physical_cores"] = len({l for l in cores.splitlines() if l and not l.startswith("#")})
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def run_reference(exe, n, steps, threads=None):
+    """one timed run of a reference build (oracle/_ref): its own loop MLUPS
+    (HARNESS line, SRC/finalize.F90's definition) and an external wall clock
+    around the whole process"""
+    env = dict(os.environ)
+    if threads:
+        env["OMP_NUM_THREADS"] = str(threads)
+    t0 = time.perf_counter()
+    r = subprocess.run([exe, str(n), str(n), "0.04096", "0.1", str(steps), "16", "-"],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, cwd="/tmp",
+                       timeout=600, env=env)
+    wall = time.perf_counter() - t0
+    m = re.search(r"loop_s=\s*([0-9.]+)\s+mlups=\s*([0-9.]+)", r.stdout)
+    if not m:
+        return None
+    return {"mlups": float(m.group(2)), "loop_s": float(m.group(1)), "process_wall_s": round(wall, 3),
+            "steps": steps, "threads": threads or 1}
+
+
+def cpu_baseline():
+    """The reference's own FUSED SINGLE LDC sources (oracle/_ref, flang -O3),
+    on a bounded sample of the same 4096^2 workload: the serial build on one
+    core, and the reference's two multicore forms on the host's physical
+    cores (at most 16, the CPU share of one GPU): its do-concurrent loops
+    mapped to OpenMP (the MULTICORE build) and its OpenMP-offload loops
+    built without a device (target regions on the host)."""
     n = 4096
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_SINGLE_FUSED_LDC")
+    host = host_info()
     if os.path.exists(exe):
-        steps = 30
-        r = subprocess.run([exe, str(n), str(n), "0.04096", "0.1", str(steps), "16", "-"],
-                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
-                           cwd="/tmp", timeout=600)
-        m = re.search(r"mlups=\s*([0-9.]+)", r.stdout)
-        if m:
-            return {"value": float(m.group(1)), "unit": "MLUPS", "cores": 1, "kind": "reference",
-                    "sample": f"4096x4096 LDC Re=1e4, {steps} steps, serial flang -O3 build "
-                              "of the reference's FUSED SINGLE sources (oracle/_ref)"}
+        serial = run_reference(exe, n, 30)
+        if serial:
+            out = {"value": serial["mlups"], "unit": "MLUPS", "cores": 1, "kind": "reference",
+                   "sample": f"4096x4096 LDC Re=1e4, {serial['steps']} steps, serial flang -O3 build "
+                             "of the reference's FUSED SINGLE sources (oracle/_ref)",
+                   "serial": serial, "host": host}
+            threads = min(16, host.get("physical_cores") or host["nproc"], host["nproc"])
+            mc = {}
+            # flang's builds of both run 100-250 times slower than its serial
+            # build on these cores (2 and 1.1 MLUPS on 16 EPYC cores against
+            # 274 serial), so their samples are 2 and 1 steps
+            for name, tag, steps in (("do_concurrent_openmp", "refmc", 2), ("openmp_target_on_host", "refomp", 1)):
+                e = os.path.join(ROOT, "oracle", "_ref", f"{tag}_SINGLE_FUSED_LDC")
+                if os.path.exists(e):
+                    mc[name] = run_reference(e, n, steps, threads)
+            out["multicore"] = mc
+            best = max([v for v in mc.values() if v] + [serial], key=lambda v: v["mlups"])
+            out["best"] = {"mlups": best["mlups"], "threads": best["threads"]}
+            return out
     # fallback: the oracle (C restatement), one core
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -103,7 +154,7 @@ def cpu_baseline(budget_s=20.0):
     o.step(steps)
     dt = time.perf_counter() - t0
     return {"value": n * n * steps / dt / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
-            "sample": f"4096x4096 LDC Re=1e4, {steps} steps, oracle C restatement"}
+            "sample": f"4096x4096 LDC Re=1e4, {steps} steps, oracle C restatement", "host": host}
 
 
 CX = (1, 1, 1, 0, -1, -1, -1, 0, 0)
@@ -273,6 +324,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-only", action="store_true", help="only the CPU baseline leg")
     ap.add_argument("--fp16-steps", type=int, default=2000)
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--fp16-only", action="store_true",
@@ -314,6 +366,9 @@ def main():
         return
     if args.fp16_only:
         print(json.dumps(fp16_leg(args.fp16_steps)))
+        return
+    if args.cpu_only:
+        print(json.dumps(cpu_baseline()))
         return
 
     n = 4096
