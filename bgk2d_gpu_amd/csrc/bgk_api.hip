@@ -467,7 +467,8 @@ int bgk_batch_create(int32_t device, double budget_bytes, int32_t max_members, b
     b->grid = cus * 8;
     b->cus = cus;
     for (int k = 0; k < 3; k++)
-        for (int v = 0; v < 6; v++) b->grid_tb[k][v] = 0;
+        for (int f = 0; f < 2; f++)
+            for (int v = 0; v < 6; v++) b->grid_tb[k][f][v] = 0;
     if (hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess) {
         batch_release(b);
         return BGK_E_HIP;
@@ -522,6 +523,12 @@ int bgk_batch_add(bgk_batch_t *b, const bgk_params_t *p, int32_t init_v, double 
         return bfail(b, BGK_E_INVALID, "batch members are FUSED, shifted, without obstacle or step, fp32/fp16/bf16");
     if (p->device != b->dev) return bfail(b, BGK_E_INVALID, "member on device %d, batch on %d", p->device, b->dev);
     if (const char *why = param_error(p)) return bfail(b, BGK_E_INVALID, "%s", why);
+    {
+        const bool narrow = p->prec == BGK_PREC_FP16 || p->prec == BGK_PREC_BF16;
+        if (narrow && p->form != p->rounding)
+            return bfail(b, BGK_E_INVALID, "16-bit batch members: form 0 with nearest-even stores or form 1 with "
+                                           "stochastic rounding");
+    }
     int tw_log2, th;
     if (!tile_shape(tile, &tw_log2, &th)) return bfail(b, BGK_E_INVALID, "unknown tile code %d", tile);
     const size_t bytes = footprint_of(p);
@@ -617,7 +624,7 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
     for (int g = 0; g < G; g++) {
         tbh[g] = bgk::TB_BATCH_H;
         if (!strip_rows[g]) continue;
-        const int slots = grid_tb(b, g / 6 - bgk::TWO_STEP, g % 6);
+        const int slots = grid_tb(b, bgk::group_slot(g) - bgk::TWO_STEP, bgk::group_fl(g), bgk::group_variant(g));
         if (strip_rows[g] < (size_t)TB_HMIN * (size_t)slots) {
             for (int mi = 0; mi < nm; mi++)
                 if (tbm[mi] && member_group(b->slots[members_slot[mi]].ctx, true) == g) tbm[mi] = 0;
@@ -715,7 +722,7 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
     int order[G], ng = 0, where[G];
     for (int g = 0; g < G; g++) {
         if (!cnt[g]) continue;
-        if (g / 6 >= bgk::TWO_STEP) {
+        if (bgk::group_slot(g) >= bgk::TWO_STEP) {
             where[g] = 0;
             load[0] += 0.5 * gnodes[g];  // two steps per pass: about half the time per update
         } else {
@@ -742,8 +749,9 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
         // the chip to the other groups' chains
         // (two-step groups: always the full grid, whose surplus blocks exit
         // at once; their strip height already sizes the work to it)
-        const bool two = g / 6 >= bgk::TWO_STEP;
-        const int full = two ? grid_tb(b, g / 6 - bgk::TWO_STEP, g % 6) : b->grid;
+        const bool two = bgk::group_slot(g) >= bgk::TWO_STEP;
+        const int full =
+            two ? grid_tb(b, bgk::group_slot(g) - bgk::TWO_STEP, bgk::group_fl(g), bgk::group_variant(g)) : b->grid;
         const int nb = !two && cnt[g] <= SMALL_GROUP ? (int)SMALL_GROUP : full;
         const int nl = two ? chunk / 2 : chunk;  // launches of the chain
         const long long key = ((long long)chunk * G + g) * 2 + (nb == full);

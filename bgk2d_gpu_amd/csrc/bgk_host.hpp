@@ -830,7 +830,7 @@ struct bgk_batch {
     static constexpr int NQ = 3;
     hipStream_t qs[NQ];
     hipEvent_t qev[NQ];
-    int cus, grid_tb[3][6];
+    int cus, grid_tb[3][2][6];
     int grid;
     hipEvent_t ev0, ev1;
     double last_ms;
@@ -925,46 +925,62 @@ inline int member_group(const bgk_ctx *c, bool tb)
 {
     int v = 0;
     while (v < 5 && bgk::BATCH_BCS[v] != c->p.bc) v++;
-    return ((tb ? bgk::TWO_STEP : 0) + member_kind(c)) * 6 + v;
+    return bgk::group_of((tb ? bgk::TWO_STEP : 0) + member_kind(c), fl_of(c), v);
 }
 
 // one step of a group's members
-template <class S>
-void launch_batch_s(bgk_batch_t *b, int g, int nb, int t, hipStream_t s)
+template <class S, int FL>
+void launch_batch_sf(bgk_batch_t *b, int g, int nb, int t, hipStream_t s)
 {
     const dim3 grid(nb), blk(256);
     const bgk::Member *dm = b->d_members;
     const int2 *dt = b->d_tiles;
     const int *meta = b->d_ntiles;
-    switch (bgk::BATCH_BCS[g % 6]) {
-    case bgk::BK_PERIODIC: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_PERIODIC>), grid, blk, 0, s, dm, dt, meta, g, t); break;
-    case bgk::BK_CHANNEL: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_CHANNEL>), grid, blk, 0, s, dm, dt, meta, g, t); break;
-    case bgk::BK_INFLOW: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_INFLOW>), grid, blk, 0, s, dm, dt, meta, g, t); break;
-    case bgk::BK_COUETTE: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_COUETTE>), grid, blk, 0, s, dm, dt, meta, g, t); break;
-    case bgk::BK_OPEN: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_OPEN>), grid, blk, 0, s, dm, dt, meta, g, t); break;
-    default: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_DRIVEN>), grid, blk, 0, s, dm, dt, meta, g, t); break;
+    switch (bgk::BATCH_BCS[bgk::group_variant(g)]) {
+    case bgk::BK_PERIODIC: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_PERIODIC, FL>), grid, blk, 0, s, dm, dt, meta, g, t); break;
+    case bgk::BK_CHANNEL: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_CHANNEL, FL>), grid, blk, 0, s, dm, dt, meta, g, t); break;
+    case bgk::BK_INFLOW: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_INFLOW, FL>), grid, blk, 0, s, dm, dt, meta, g, t); break;
+    case bgk::BK_COUETTE: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_COUETTE, FL>), grid, blk, 0, s, dm, dt, meta, g, t); break;
+    case bgk::BK_OPEN: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_OPEN, FL>), grid, blk, 0, s, dm, dt, meta, g, t); break;
+    default: hipLaunchKernelGGL((bgk::k_batch<S, bgk::BK_DRIVEN, FL>), grid, blk, 0, s, dm, dt, meta, g, t); break;
     }
 }
 template <class S>
-void launch_batch_tb(bgk_batch_t *b, int g, int nb, int t2, hipStream_t s)
+void launch_batch_s(bgk_batch_t *b, int g, int nb, int t, hipStream_t s)
+{
+    if (bgk::group_fl(g))
+        launch_batch_sf<S, 1>(b, g, nb, t, s);
+    else
+        launch_batch_sf<S, 0>(b, g, nb, t, s);
+}
+template <class S, int FL>
+void launch_batch_tbf(bgk_batch_t *b, int g, int nb, int t2, hipStream_t s)
 {
     const dim3 grid(nb), blk(bgk::TB_TW);
     const bgk::Member *dm = b->d_members;
     const int2 *dt = b->d_tiles;
     const int *meta = b->d_ntiles;
-    switch (bgk::BATCH_BCS[g % 6]) {
-    case bgk::BK_PERIODIC: hipLaunchKernelGGL((bgk::k_batch_tb<S, bgk::BK_PERIODIC>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
-    case bgk::BK_CHANNEL: hipLaunchKernelGGL((bgk::k_batch_tb<S, bgk::BK_CHANNEL>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
-    case bgk::BK_INFLOW: hipLaunchKernelGGL((bgk::k_batch_tb<S, bgk::BK_INFLOW>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
-    case bgk::BK_COUETTE: hipLaunchKernelGGL((bgk::k_batch_tb<S, bgk::BK_COUETTE>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
-    case bgk::BK_OPEN: hipLaunchKernelGGL((bgk::k_batch_tb<S, bgk::BK_OPEN>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
-    default: hipLaunchKernelGGL((bgk::k_batch_tb<S, bgk::BK_DRIVEN>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
+    switch (bgk::BATCH_BCS[bgk::group_variant(g)]) {
+    case bgk::BK_PERIODIC: hipLaunchKernelGGL((bgk::k_batch_tb<S, bgk::BK_PERIODIC, FL>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
+    case bgk::BK_CHANNEL: hipLaunchKernelGGL((bgk::k_batch_tb<S, bgk::BK_CHANNEL, FL>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
+    case bgk::BK_INFLOW: hipLaunchKernelGGL((bgk::k_batch_tb<S, bgk::BK_INFLOW, FL>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
+    case bgk::BK_COUETTE: hipLaunchKernelGGL((bgk::k_batch_tb<S, bgk::BK_COUETTE, FL>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
+    case bgk::BK_OPEN: hipLaunchKernelGGL((bgk::k_batch_tb<S, bgk::BK_OPEN, FL>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
+    default: hipLaunchKernelGGL((bgk::k_batch_tb<S, bgk::BK_DRIVEN, FL>), grid, blk, 0, s, dm, dt, meta, g, t2); break;
     }
+}
+template <class S>
+void launch_batch_tb(bgk_batch_t *b, int g, int nb, int t2, hipStream_t s)
+{
+    if (bgk::group_fl(g))
+        launch_batch_tbf<S, 1>(b, g, nb, t2, s);
+    else
+        launch_batch_tbf<S, 0>(b, g, nb, t2, s);
 }
 // launch t of a group's chain (a two-step group's chain has chunk / 2)
 inline void launch_batch(bgk_batch_t *b, int g, int nb, int t, hipStream_t s)
 {
-    switch (g / 6) {
+    switch (bgk::group_slot(g)) {
     case bgk::MK_F16: launch_batch_s<_Float16>(b, g, nb, t, s); break;
     case bgk::MK_BF16: launch_batch_s<bgk::bf16_t>(b, g, nb, t, s); break;
     case bgk::TWO_STEP + bgk::MK_F32: launch_batch_tb<float>(b, g, nb, t, s); break;
@@ -975,31 +991,36 @@ inline void launch_batch(bgk_batch_t *b, int g, int nb, int t, hipStream_t s)
 }
 // blocks of the two-step batch kernel the device holds at once: its grid,
 // since a persistent block that starts late would run its whole share late
-template <class S>
-int occupancy_tb(int v, int *per_cu)
+template <class S, int FL>
+int occupancy_tbf(int v, int *per_cu)
 {
     switch (bgk::BATCH_BCS[v]) {
-    case bgk::BK_PERIODIC: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_PERIODIC>, bgk::TB_TW, 0);
-    case bgk::BK_CHANNEL: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_CHANNEL>, bgk::TB_TW, 0);
-    case bgk::BK_INFLOW: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_INFLOW>, bgk::TB_TW, 0);
-    case bgk::BK_COUETTE: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_COUETTE>, bgk::TB_TW, 0);
-    case bgk::BK_OPEN: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_OPEN>, bgk::TB_TW, 0);
-    default: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_DRIVEN>, bgk::TB_TW, 0);
+    case bgk::BK_PERIODIC: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_PERIODIC, FL>, bgk::TB_TW, 0);
+    case bgk::BK_CHANNEL: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_CHANNEL, FL>, bgk::TB_TW, 0);
+    case bgk::BK_INFLOW: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_INFLOW, FL>, bgk::TB_TW, 0);
+    case bgk::BK_COUETTE: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_COUETTE, FL>, bgk::TB_TW, 0);
+    case bgk::BK_OPEN: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_OPEN, FL>, bgk::TB_TW, 0);
+    default: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, bgk::k_batch_tb<S, bgk::BK_DRIVEN, FL>, bgk::TB_TW, 0);
     }
+}
+template <class S>
+int occupancy_tb(int v, int fl, int *per_cu)
+{
+    return fl ? occupancy_tbf<S, 1>(v, per_cu) : occupancy_tbf<S, 0>(v, per_cu);
 }
 // blocks of the two-step batch kernel of storage kind `kind`, variant v,
 // that the device holds at once: its grid, since a persistent block that
 // starts late would run its whole share late
-inline int grid_tb(bgk_batch_t *b, int kind, int v)
+inline int grid_tb(bgk_batch_t *b, int kind, int fl, int v)
 {
-    if (b->grid_tb[kind][v] > 0) return b->grid_tb[kind][v];
+    if (b->grid_tb[kind][fl][v] > 0) return b->grid_tb[kind][fl][v];
     int per_cu = 0;
-    const int e = kind == bgk::MK_F16 ? occupancy_tb<_Float16>(v, &per_cu)
-                  : kind == bgk::MK_BF16 ? occupancy_tb<bgk::bf16_t>(v, &per_cu)
-                                         : occupancy_tb<float>(v, &per_cu);
+    const int e = kind == bgk::MK_F16 ? occupancy_tb<_Float16>(v, fl, &per_cu)
+                  : kind == bgk::MK_BF16 ? occupancy_tb<bgk::bf16_t>(v, fl, &per_cu)
+                                         : occupancy_tb<float>(v, fl, &per_cu);
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
-    b->grid_tb[kind][v] = per_cu * b->cus;
-    return b->grid_tb[kind][v];
+    b->grid_tb[kind][fl][v] = per_cu * b->cus;
+    return b->grid_tb[kind][fl][v];
 }
 
 inline bool tile_shape(int code, int *tw_log2, int *th)
@@ -1080,11 +1101,11 @@ extern template int run_t<bgk::bf16_t, float>(bgk_ctx *, int);
 extern template int enqueue_t<bgk::bf16_t, float>(bgk_ctx *, int);
 extern template void launch_batch_s<float>(bgk_batch_t *, int, int, int, hipStream_t);
 extern template void launch_batch_tb<float>(bgk_batch_t *, int, int, int, hipStream_t);
-extern template int occupancy_tb<float>(int, int *);
+extern template int occupancy_tb<float>(int, int, int *);
 extern template void launch_batch_s<_Float16>(bgk_batch_t *, int, int, int, hipStream_t);
 extern template void launch_batch_tb<_Float16>(bgk_batch_t *, int, int, int, hipStream_t);
-extern template int occupancy_tb<_Float16>(int, int *);
+extern template int occupancy_tb<_Float16>(int, int, int *);
 extern template void launch_batch_s<bgk::bf16_t>(bgk_batch_t *, int, int, int, hipStream_t);
 extern template void launch_batch_tb<bgk::bf16_t>(bgk_batch_t *, int, int, int, hipStream_t);
-extern template int occupancy_tb<bgk::bf16_t>(int, int *);
+extern template int occupancy_tb<bgk::bf16_t>(int, int, int *);
 }  // namespace bgk_host

@@ -34,11 +34,18 @@ enum MemberKind {
 
 // boundary variants the batch kernels serve, in group order
 constexpr int BATCH_BCS[6] = {BK_DRIVEN, BK_PERIODIC, BK_CHANNEL, BK_INFLOW, BK_COUETTE, BK_OPEN};
-// group = kind * 6 + index of the variant in BATCH_BCS; kinds TWO_STEP +
-// MK_* hold the members of that storage kind that advance two steps per
-// launch in this chunk
+// group = (kind slot * 2 + fl) * 6 + index of the variant in BATCH_BCS:
+// kind slots TWO_STEP + MK_* hold the members of that storage kind that
+// advance two steps per launch in this chunk; fl is the kernel instance's
+// form/rounding flag (Cfg in bgk_kernels.hpp): 0 the reference's form
+// with nearest-even stores, 1 the deviation form with stochastic rounding
+// of 16-bit stores (the serving settings)
 constexpr int TWO_STEP = 3;
-constexpr int NGROUPS = 6 * 6;
+constexpr int NGROUPS = 6 * 2 * 6;
+__host__ __device__ constexpr int group_of(int slot, int fl, int v) { return (slot * 2 + fl) * 6 + v; }
+__host__ __device__ constexpr int group_slot(int g) { return g / 12; }
+__host__ __device__ constexpr int group_fl(int g) { return (g / 6) % 2; }
+__host__ __device__ constexpr int group_variant(int g) { return g % 6; }
 // largest strip height of the two-step batch kernel
 constexpr int TB_BATCH_H = 64;
 
@@ -71,18 +78,11 @@ struct Member {
 // meta: [0, NGROUPS) task counts of the groups; [NGROUPS, 2 NGROUPS) their
 // first task in the table.  A task (member, first tile) covers the tiles
 // first .. min(first + TASK_TILES, ntiles) - 1.
-// Eight waves per SIMD asked of the compiler for the light variants (at
-// most 64 VGPRs; the 16-bit driven kernel needed 69, and spills 8 bytes
-// under the bound): 16-bit members ran 5-16 % faster.  The inflow, couette
-// and open variants would spill 28-208 bytes, so they keep the default.
-template <int BC>
-constexpr int batch_waves()
-{
-    return (BC == BK_INFLOW || BC == BK_COUETTE || BC == BK_OPEN) ? 1 : 8;
-}
-
-template <class S, int BC>
-__global__ __launch_bounds__(256, batch_waves<BC>()) void k_batch(const Member *__restrict__ M, const int2 *__restrict__ tasks,
+// No occupancy bound: under a bound of 8 waves per SIMD (64 VGPRs) the
+// compiler spilled 100-270 bytes per lane and a mixed stream of 128..1024
+// lattices ran at 20 GLUPS instead of 34.
+template <class S, int BC, int FL>
+__global__ __launch_bounds__(256) void k_batch(const Member *__restrict__ M, const int2 *__restrict__ tasks,
                                                const int *__restrict__ meta, int group, int t)
 {
     const int n = meta[group];
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256, batch_waves<BC>()) void k_batch(const Member *
         L.jmax = 0;
         L.stepx = 0;
         L.stepy = 0;
-        const Phys<float> P = d.P;
+        const Phys<float> P = fixed_flags<S, float, BC, FL>(d.P);
         const int tiles_x = d.tiles_x, tw_log2 = d.tw_log2, th = d.th;
         const int tend = min(tl.y + TASK_TILES, d.ntiles);
         const int lane_x = (int)threadIdx.x & ((1 << tw_log2) - 1);
@@ -128,9 +128,9 @@ __global__ __launch_bounds__(256, batch_waves<BC>()) void k_batch(const Member *
             if (i > L.l || j > L.m) continue;
             // one node (fp32) or the pair (i, i+1) (16-bit kinds), as bgk_run's kernels
             if constexpr (sizeof(S) == 2)
-                fused_pair<S, float, BC, false, -1>(L, P, i, j);
+                fused_pair<S, float, BC, false, FL>(L, P, i, j);
             else
-                fused_node<S, float, BC, false, -1>(L, P, i, j);
+                fused_node<S, float, BC, false, FL>(L, P, i, j);
         }
     }
 }
@@ -179,7 +179,7 @@ __device__ __forceinline__ Phys<float> cld_phys(const Phys<float> *p)
     return r;
 }
 
-template <class S, int BC>
+template <class S, int BC, int FL>
 __global__ __launch_bounds__(TB_TW) void k_batch_tb(const Member *__restrict__ M, const int2 *__restrict__ tasks,
                                                   const int *__restrict__ meta, int group, int t2)
 {
@@ -213,10 +213,10 @@ __global__ __launch_bounds__(TB_TW) void k_batch_tb(const Member *__restrict__ M
         L.jmax = 0;
         L.stepx = 0;
         L.stepy = 0;
-        const Phys<float> P = cld_phys(&d->P);
+        const Phys<float> P = fixed_flags<S, float, BC, FL>(cld_phys(&d->P));
         const int tiles_x = cld(&d->tiles_x);
         const int th = cld(&d->th), j0 = (tile / tiles_x) * th + 1;
-        tb_block<S, float, BC, false, -1>(L, P, tile % tiles_x, j0, min(j0 + th - 1, L.m));
+        tb_block<S, float, BC, false, FL>(L, P, tile % tiles_x, j0, min(j0 + th - 1, L.m));
     }
 }
 
