@@ -221,34 +221,84 @@ def precision_study(n=8192, steps=5000, amp=0.01, svisc=0.1, init_v=0):
     return res
 
 
-def serving_leg(n=48, seed=20261203, side=(128, 2048), steps=(1000, 10000), chunk=100,
-                budget=200e9):
-    """Mixed request stream (cavity / Poiseuille / Taylor-Green, sides and
-    Re log-uniform, steps uniform, FP32 or FP16 storage, Poisson arrivals at
-    200 req/s), served by continuous batching (every admitted request each
-    chunk; and shortest remaining work first over at least 2^22 nodes) and
-    by sequential FIFO solves; p50/p99 time-to-solution and aggregate
-    MLUPS."""
+def kernel_ceilings(steps=400):
+    """MLUPS of bgk_run alone on a 4096^2 lattice of each serving kind and
+    storage kind (the kernels a sequential solve of that size uses): the
+    rate a perfectly packed batch could reach"""
+    from bgk2d_gpu_amd import serve
+    out = {}
+    for kind in serve.KINDS:
+        for prec in ("single", "fp16"):
+            r = serve.Request(0, kind, 4096, 1000.0, steps, prec)
+            lx, ly, bc, svisc, u0, amp = r.geometry()
+            g = gpu_solve_xy(lx, ly, prec, bc, steps, 40, svisc, u0, amp)
+            out[f"{kind}/{prec}"] = round(lx * ly * steps / g / 1e6, 1)
+    return out
+
+
+def gpu_solve_xy(lx, ly, prec, bc, steps, warmup, svisc, u0, amp):
+    """wall seconds of bgk_run(steps) on an lx x ly lattice (serving settings)"""
+    from bgk2d_gpu_amd import Solver
+    narrow = prec in ("fp16", "bf16")
+    s = Solver(lx, ly, bc=bc, impl="fused", prec=prec, svisc=svisc, u0=u0, amplitude=amp,
+               scale=1024.0 if narrow else 1.0, form=1 if narrow else 0, rounding=1 if narrow else 0)
+    s.run(warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    s.run(steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    s.close()
+    return wall
+
+
+def serving_leg(n=48, seed=20261203, side=(128, 4096), steps=(1000, 50000), chunk=100,
+                budget=200e9, fill=1 << 23):
+    """The serving configuration: a seeded Poisson stream (200 req/s) of
+    cavity / Poiseuille / Taylor-Green solves, sides log-uniform in
+    128..4096, Re log-uniform in 100..1e4, steps uniform in 1k..50k, FP32
+    or FP16 storage.  Served by continuous batching with FIFO admission,
+    each chunk advancing the admitted requests with the least remaining
+    work whose lattices hold at least `fill` nodes (2^23: the p50 below the
+    sequential one at most of the all-member throughput); by the same loop
+    advancing every admitted request each chunk (the most throughput); and
+    by sequential FIFO solves with the same kernels.  Reports p50/p99
+    time-to-solution, aggregate MLUPS and their ratios, and the batched
+    aggregate as a fraction of the stream's kernel ceiling (its lattice
+    updates at the bgk_run rate of each request's kind at 4096^2)."""
     from bgk2d_gpu_amd import serve
     reqs = serve.make_stream(n, seed=seed, side=side, steps=steps)
     tiles = {p: serve.autotune(p, 1024) for p in ("single", "fp16")}
+    ceil = kernel_ceilings()
     bo, bs = serve.serve_batched(reqs, budget_bytes=budget, chunk=chunk, check_every=1000,
-                                 tiles=tiles)
-    fo, fs = serve.serve_batched(reqs, budget_bytes=budget, chunk=chunk, check_every=1000,
-                                 tiles=tiles, fill_nodes=1 << 22)
+                                 tiles=tiles, fill_nodes=fill)
+    ao, as_ = serve.serve_batched(reqs, budget_bytes=budget, chunk=chunk, check_every=1000,
+                                  tiles=tiles)
     so, ss = serve.serve_sequential(reqs, check_every=1000)
     b, q = serve.summarize(reqs, bo, bs), serve.summarize(reqs, so, ss)
-    f = serve.summarize(reqs, fo, fs)
-    same = all(bo[r.rid].status == so[r.rid].status for r in reqs)
-    diag_same = all(bo[r.rid].diag == so[r.rid].diag for r in reqs if bo[r.rid].status == "ok")
+    a = serve.summarize(reqs, ao, as_)
+    # the stream's kernel ceiling: its updates at each kind's bgk_run rate
+    ok = [r for r in reqs if so[r.rid].status == "ok"]
+    t_ceil = sum(r.lups / (ceil[f"{r.kind}/{r.prec}"] * 1e6) for r in ok)
+    ceiling = sum(r.lups for r in ok) / t_ceil / 1e6 if t_ceil else None
     return {"stream": f"{n} requests, seed {seed}, sides log-uniform {side[0]}..{side[1]}, "
                       f"Re log-uniform 100..1e4, steps uniform {steps[0]}..{steps[1]}, "
                       "FP32/FP16 50/50, Poisson 200 req/s",
-            "batched": b, "sequential": q,
-            "batched_shortest_first": dict(f, fill_nodes=1 << 22,
-                                           same_outcomes=all(fo[r.rid].status == so[r.rid].status for r in reqs)),
+            "lattice_updates": int(sum(r.lups for r in reqs)),
+            "batched": dict(b, policy=f"shortest remaining work first, >= {fill} nodes per chunk"),
+            "batched_all_members": a, "sequential": q,
             "aggregate_mlups_ratio": round(b["mlups"] / q["mlups"], 3),
-            "tiles": tiles, "same_outcomes": same, "same_diagnostics": diag_same}
+            "all_members_mlups_ratio": round(a["mlups"] / q["mlups"], 3),
+            "batched_p50_over_sequential_p50": round(b["p50_s"] / q["p50_s"], 3),
+            "all_members_p50_over_sequential_p50": round(a["p50_s"] / q["p50_s"], 3),
+            "kernel_ceilings_4096": ceil,
+            "stream_kernel_ceiling_mlups": round(ceiling, 1) if ceiling else None,
+            "batched_over_ceiling": round(b["mlups"] / ceiling, 3) if ceiling else None,
+            "all_members_over_ceiling": round(a["mlups"] / ceiling, 3) if ceiling else None,
+            "tiles": tiles,
+            "same_outcomes": all(bo[r.rid].status == so[r.rid].status == ao[r.rid].status for r in reqs),
+            "same_diagnostics": all(bo[r.rid].diag == so[r.rid].diag == ao[r.rid].diag
+                                    for r in reqs if so[r.rid].status == "ok")}
 
 
 def c4_leg(steps=2000):
@@ -332,8 +382,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--serve-n", type=int, default=48)
-    ap.add_argument("--serve-full", action="store_true",
-                    help="serving leg only, on the full C5 distributions (sides 128..4096, steps 1k..50k)")
+    ap.add_argument("--serve-full", action="store_true", help="the serving leg only (same as --serve-only)")
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--c4-only", action="store_true")
     ap.add_argument("--c4-steps", type=int, default=2000)
@@ -359,7 +408,7 @@ def main():
         print(json.dumps(serving_leg(args.serve_n)))
         return
     if args.serve_full:
-        print(json.dumps(serving_leg(args.serve_n, side=(128, 4096), steps=(1000, 50000))))
+        print(json.dumps(serving_leg(args.serve_n)))
         return
     if args.c4_only:
         print(json.dumps(c4_leg(args.c4_steps)))
