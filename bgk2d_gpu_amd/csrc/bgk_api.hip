@@ -7,7 +7,39 @@
 // bgk_kernels.hpp in the reference's phase order.
 #include "bgk_host.hpp"
 
+#include <mutex>
+
 using namespace bgk_host;
+
+// ---------------------------------------------------------------------
+// Memory of bgk_init_noalloc: the trig tables of init in `scratch`, one
+// copy per GPU (held until the process ends), staged from h_stage.  The
+// array's name is a host-side stand-in: host code uses only the device
+// address hipGetSymbolAddress returns, kept per device in scratch_ptr.
+// Calls serialize on scratch_mu, which is held until the kernel that
+// reads the tables has ended.
+static __device__ __align__(256) unsigned char scratch[BGK_INIT_SCRATCH_BYTES];
+static unsigned char h_stage[BGK_INIT_SCRATCH_BYTES];
+static std::mutex scratch_mu;
+constexpr int SCRATCH_DEVICES = 64;
+static void *scratch_ptr[SCRATCH_DEVICES];
+
+// device address of `scratch` on the current device (c's), looked up once
+// per device; call with scratch_mu held
+static int scratch_on(bgk_ctx *c, void **out)
+{
+    if (c->dev < 0 || c->dev >= SCRATCH_DEVICES)
+        return fail(c, BGK_E_INVALID, "bgk_init_noalloc: device %d beyond %d", c->dev, SCRATCH_DEVICES - 1);
+    if (!scratch_ptr[c->dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(c, BGK_E_HIP, "bgk_init_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+        scratch_ptr[c->dev] = ptr;
+    }
+    *out = scratch_ptr[c->dev];
+    return BGK_OK;
+}
 
 
 // =====================================================================
@@ -227,6 +259,25 @@ int bgk_init(bgk_ctx_t *c, int32_t init_v, double amplitude)
     if (r) return r;
     if (init_v < 0 || init_v > 5) return fail(c, BGK_E_INVALID, "Initial condition out of range[0,5] %d", init_v);
     r = DISPATCH(c, init_t, amplitude, (int)init_v);
+    if (r == BGK_OK) c->inited = true;
+    return r;
+}
+
+int bgk_init_noalloc(bgk_ctx_t *c, int32_t init_v, double amplitude)
+{
+    int r = check_ctx(c);
+    if (r) return r;
+    if (init_v < 0 || init_v > 5) return fail(c, BGK_E_INVALID, "Initial condition out of range[0,5] %d", init_v);
+    const size_t need = init_table_bytes(c), per_node = need / ((size_t)c->l + c->m + 4);
+    if (need > (size_t)BGK_INIT_SCRATCH_BYTES)
+        return fail(c, BGK_E_INVALID,
+                    "bgk_init_noalloc: trig tables of %zu bytes exceed the %d-byte scratch array "
+                    "(lx + ly + 4 <= %zu for this compute kind); use bgk_init",
+                    need, BGK_INIT_SCRATCH_BYTES, (size_t)BGK_INIT_SCRATCH_BYTES / per_node);
+    std::lock_guard<std::mutex> hold(scratch_mu);
+    void *d = nullptr;
+    if ((r = scratch_on(c, &d))) return r;
+    r = DISPATCH(c, init_static_t, amplitude, (int)init_v, d, (void *)h_stage);
     if (r == BGK_OK) c->inited = true;
     return r;
 }

@@ -577,37 +577,72 @@ inline void host_sincos<float>(float arg, float *s, float *c) { sincosf(arg, s, 
 template <>
 inline void host_sincos<double>(double arg, double *s, double *c) { sincos(arg, s, c); }
 
-template <class S, class C>
-int init_t(bgk_ctx *c, double amp, int init_v)
+// the trig tables of init: sin and cos of 2 pi x over nodes 0..l+1, then
+// of 2 pi y over nodes 0..m+1 (2 (l + m + 4) elements of the compute kind)
+inline size_t init_table_bytes(const bgk_ctx *c)
 {
-    const bool tgv = c->p.bc == BGK_BC_PERIODIC;
+    const size_t ck = (c->p.prec == BGK_PREC_DOUBLE || c->p.prec == BGK_PREC_MIXED2) ? 8 : 4;
+    return 2 * ((size_t)c->l + (size_t)c->m + 4) * ck;
+}
+
+template <class C>
+void init_tables(const bgk_ctx *c, C *h)
+{
     const int nx = c->l + 2, ny = c->m + 2;
-    std::vector<C> tsx(nx), tcx(nx), tsy(ny), tcy(ny);
     // SRC/init.F90:88-111: pi is a default-real literal in kind mykind,
     // x = (real(i)-0.5d0)/real(l) evaluated in double, sin/cos in mykind
     const C pi = (C)(float)3.141592653589793238462643383279;
     for (int i = 0; i < nx; i++) {
         C x = (C)(((double)(C)i - 0.5) / (double)(C)c->l);
-        host_sincos<C>((C)2 * pi * x, &tsx[i], &tcx[i]);
+        host_sincos<C>((C)2 * pi * x, &h[i], &h[nx + i]);
     }
     for (int j = 0; j < ny; j++) {
         C y = (C)(((double)(C)j - 0.5) / (double)(C)c->m);
-        host_sincos<C>((C)2 * pi * y, &tsy[j], &tcy[j]);
+        host_sincos<C>((C)2 * pi * y, &h[2 * nx + j], &h[2 * nx + ny + j]);
     }
-    C *d = nullptr;
-    HIPCHK(c, hipMalloc(&d, sizeof(C) * 2 * (nx + ny)));
-    HIPCHK(c, hipMemcpyAsync(d, tsx.data(), sizeof(C) * nx, hipMemcpyHostToDevice, c->s));
-    HIPCHK(c, hipMemcpyAsync(d + nx, tcx.data(), sizeof(C) * nx, hipMemcpyHostToDevice, c->s));
-    HIPCHK(c, hipMemcpyAsync(d + 2 * nx, tsy.data(), sizeof(C) * ny, hipMemcpyHostToDevice, c->s));
-    HIPCHK(c, hipMemcpyAsync(d + 2 * nx + ny, tcy.data(), sizeof(C) * ny, hipMemcpyHostToDevice, c->s));
-    Lat L = lat(c);
-    Phys<C> P = phys<C>(c);
-    hipLaunchKernelGGL((k_init<S, C>), grid2(nx, ny, B2), B2, 0, c->s, L, P, d, d + nx, d + 2 * nx,
-                       d + 2 * nx + ny, amp, tgv ? (init_v == 1 ? 2 : 1) : 0);
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    HIPCHK(c, hipFree(d));
+}
+
+// uploads the tables h into the device buffer d, launches k_init and
+// waits for it (h and d are free again on return, also on failure)
+template <class S, class C>
+int init_launch(bgk_ctx *c, double amp, int init_v, const C *h, C *d)
+{
+    const bool tgv = c->p.bc == BGK_BC_PERIODIC;
+    const int nx = c->l + 2, ny = c->m + 2;
+    hipError_t e = hipMemcpyAsync(d, h, init_table_bytes(c), hipMemcpyHostToDevice, c->s);
+    if (e == hipSuccess) {
+        Lat L = lat(c);
+        Phys<C> P = phys<C>(c);
+        hipLaunchKernelGGL((k_init<S, C>), grid2(nx, ny, B2), B2, 0, c->s, L, P, d, d + nx, d + 2 * nx,
+                           d + 2 * nx + ny, amp, tgv ? (init_v == 1 ? 2 : 1) : 0);
+        e = hipGetLastError();
+    }
+    const hipError_t w = hipStreamSynchronize(c->s);
+    if (e == hipSuccess) e = w;
+    if (e != hipSuccess) return fail(c, BGK_E_HIP, "init: %s", hipGetErrorString(e));
     return BGK_OK;
+}
+
+template <class S, class C>
+int init_t(bgk_ctx *c, double amp, int init_v)
+{
+    std::vector<C> h(init_table_bytes(c) / sizeof(C));
+    init_tables<C>(c, h.data());
+    C *d = nullptr;
+    HIPCHK(c, hipMalloc(&d, init_table_bytes(c)));
+    const int r = init_launch<S, C>(c, amp, init_v, h.data(), d);
+    const hipError_t e = hipFree(d);
+    if (r == BGK_OK && e != hipSuccess) return fail(c, BGK_E_HIP, "init: %s", hipGetErrorString(e));
+    return r;
+}
+
+// init with caller-provided tables: hstage host memory and dscratch device
+// memory of init_table_bytes(c) bytes each (no allocation)
+template <class S, class C>
+int init_static_t(bgk_ctx *c, double amp, int init_v, void *dscratch, void *hstage)
+{
+    init_tables<C>(c, (C *)hstage);
+    return init_launch<S, C>(c, amp, init_v, (const C *)hstage, (C *)dscratch);
 }
 
 template <class S, class C>

@@ -23,7 +23,7 @@ ERRORS = {-1: "invalid argument", -2: "out of memory", -3: "HIP error", -4: "div
 # every entry point declared in include/bgk2d.h
 EXPORTS = ("bgk_create", "bgk_destroy", "bgk_last_error", "bgk_version", "bgk_constants",
            "bgk_build_obs", "bgk_set_obstacle", "bgk_get_obstacle", "bgk_init",
-           "bgk_boundaries", "bgk_propagation", "bgk_collision", "bgk_run", "bgk_sync",
+           "bgk_init_noalloc", "bgk_boundaries", "bgk_propagation", "bgk_collision", "bgk_run", "bgk_sync",
            "bgk_diagno", "bgk_varm", "bgk_prof_i", "bgk_prof_j", "bgk_probe", "bgk_draglift",
            "bgk_fields", "bgk_check",
            "bgk_download", "bgk_upload", "bgk_storage_bytes", "bgk_stream", "bgk_last_run_ms",
@@ -73,6 +73,7 @@ def load(path=None):
         "bgk_set_obstacle": ([V, P(I32)], C.c_int),
         "bgk_get_obstacle": ([V, P(I32), P(I32)], C.c_int),
         "bgk_init": ([V, I32, D], C.c_int),
+        "bgk_init_noalloc": ([V, I32, D], C.c_int),
         "bgk_boundaries": ([V], C.c_int),
         "bgk_propagation": ([V], C.c_int),
         "bgk_collision": ([V, I32], C.c_int),

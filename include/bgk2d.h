@@ -162,6 +162,21 @@ int bgk_get_obstacle(bgk_ctx_t *ctx, int32_t *obs, int32_t bbox[4]);
  * cos 4 pi y), which launches no acoustic transient.                    */
 int bgk_init(bgk_ctx_t *ctx, int32_t init_v, double amplitude);
 
+/* bgk_init that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory at the point of the
+ * call (code under HIP graph capture, serving loops that recycle a
+ * context for the next request, harnesses that forbid hipMalloc).
+ * bgk_init builds its trig tables (2 (lx + ly + 4) elements of the compute
+ * kind) in a hipMalloc'd buffer; this entry point uses a static device
+ * array of BGK_INIT_SCRATCH_BYTES per GPU instead (held until the process
+ * ends) and gives the same populations bit for bit.  Lattices whose
+ * tables exceed it (lx + ly + 4 > 131072 for fp32 compute, > 65536 for
+ * fp64) are refused with BGK_E_INVALID.  Calls on every GPU serialize on
+ * one host lock, held until the kernel that reads the array has ended, so
+ * calls that overlap wait rather than race; returns after the kernel. */
+#define BGK_INIT_SCRATCH_BYTES (1 << 20)
+int bgk_init_noalloc(bgk_ctx_t *ctx, int32_t init_v, double amplitude);
+
 /* The three per-step phases of SRC/bgk2d.F90:100-102, each a kernel
  * launch on the context stream with the reference's halo semantics.     */
 int bgk_boundaries(bgk_ctx_t *ctx);
