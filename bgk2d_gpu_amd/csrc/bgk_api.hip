@@ -816,11 +816,17 @@ int bgk_batch_run(bgk_batch_t *b, int32_t chunk, int32_t *finished)
             hipGraphExec_t ge;
             auto it = b->graphs.find(key);
             if (it == b->graphs.end()) {
+                // nothing between the begin and the end of the capture
+                // returns early, so no error leaves qs capturing
                 BHIPCHK(b, hipStreamBeginCapture(qs, hipStreamCaptureModeThreadLocal));
                 for (int tt = 0; tt < nl; tt++) launch_batch(b, g, nb, tt, qs);
                 hipGraph_t gr = nullptr;
-                BHIPCHK(b, hipStreamEndCapture(qs, &gr));
-                hipError_t e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+                hipError_t e = hipStreamEndCapture(qs, &gr);
+                if (e != hipSuccess) {
+                    if (gr) (void)hipGraphDestroy(gr);
+                    return bfail(b, BGK_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+                }
+                e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(gr);
                 if (e != hipSuccess) return bfail(b, BGK_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
                 b->graphs[key] = ge;

@@ -514,16 +514,17 @@ int graph_for(bgk_ctx *c, hipGraphExec_t *out)
     }
     const int cur0 = c->cur, rcur0 = c->rcur;
     HIPCHK(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+    // the capture ends whatever launch_steps returns
     int r = launch_steps<S, C>(c, GRAPH_STEPS);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->s, &g);
     c->cur = cur0;
     c->rcur = rcur0;
-    if (r != BGK_OK) {
+    if (r != BGK_OK || e != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
-        return r;
+        if (r != BGK_OK) return r;
+        return fail(c, BGK_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
     }
-    if (e != hipSuccess) return fail(c, BGK_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
     hipGraphExec_t x = nullptr;
     e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);

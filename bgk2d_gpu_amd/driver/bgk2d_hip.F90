@@ -144,6 +144,12 @@ program bgk2d_hip
   case default; error stop 'unknown implementation'
   end select
   call read_parameters_kind()
+  ! save.*.bin / restore.bin hold the storage kind's raw values in the
+  ! layout of SRC/save_raw.F90, which has no field for a scale: with 16-bit
+  ! storage they are written or read at scale 1 only
+  if ((p%prec == BGK_PREC_FP16 .or. p%prec == BGK_PREC_BF16) .and. scale /= 1.0d0 .and. &
+      (irestart == 1 .or. (itsave > 0 .and. itsave <= itfin))) &
+     error stop 'scale /= 1: save/restore files hold 16-bit populations at scale 1 only'
   p%svisc = svisc
   p%u0 = u0
   p%cte_s = cteS

@@ -73,6 +73,15 @@ def test_driver_rejects_unknown_case_before_gpu(tmp_path):
     assert r.returncode != 0 and "unknown case_name" in r.stdout
 
 
+@pytest.mark.parametrize("kw", [dict(itsave=300), dict(itsave=1000, irestart=1)])
+def test_driver_refuses_scaled_16bit_save_or_restore_before_gpu(tmp_path, kw):
+    """the save/restore layout has no scale field: FP16/BF16 with scale != 1
+    cannot write or read it"""
+    write_input(str(tmp_path), prec="FP16", extra="scale=1024.d0", **kw)
+    r = run_driver(str(tmp_path))
+    assert r.returncode != 0 and "scale 1 only" in r.stdout
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("phases", [0, 1])
 @pytest.mark.parametrize("case,prec,impl,extra", [("LDC", "SINGLE", "FUSED", ""),
