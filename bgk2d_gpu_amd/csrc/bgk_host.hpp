@@ -349,6 +349,11 @@ int launch_step(bgk_ctx *c)
 // where it beats one step per pass (Taylor-Green 8192^2: FP32 122 k against
 // 75 k MLUPS, FP16 157 k against 145 k).  Needs size_tb first.
 constexpr int TB_HMIN = 8;
+// Blocks per CU that the two-step grids are sized for, at most: the 16-bit
+// instances that the occupancy query allows seven of ran slower sized for
+// seven than for six (Taylor-Green 8192^2 FP16: 158 k against 161 k MLUPS),
+// and six is what the fp32 instances hold anyway (their LDS ring).
+constexpr int TB_MAX_PER_CU = 6;
 inline bool use_tb(const bgk_ctx *c)
 {
     if (!foldable(c)) return false;
@@ -400,6 +405,7 @@ int size_tb(bgk_ctx *c)
     });
     if (e != hipSuccess) return fail(c, BGK_E_HIP, "hipOccupancyMaxActiveBlocksPerMultiprocessor: %s", hipGetErrorString(e));
     HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->dev));
+    if (per_cu > TB_MAX_PER_CU) per_cu = TB_MAX_PER_CU;
     if (per_cu < 1) per_cu = 1;
     if (cus < 1) cus = 1;
     const long long slots = (long long)per_cu * cus;
@@ -1055,6 +1061,7 @@ inline int grid_tb(bgk_batch_t *b, int kind, int fl, int v)
                   : kind == bgk::MK_BF16 ? occupancy_tb<bgk::bf16_t>(v, fl, &per_cu)
                                          : occupancy_tb<float>(v, fl, &per_cu);
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    if (per_cu > TB_MAX_PER_CU) per_cu = TB_MAX_PER_CU;
     b->grid_tb[kind][fl][v] = per_cu * b->cus;
     return b->grid_tb[kind][fl][v];
 }
