@@ -343,6 +343,63 @@ def c4_leg(steps=2000):
     return out
 
 
+def c4_shedding(steps=400000, every=256):
+    """The C4 channel (geometry of c4_leg) run long enough to shed: from rest,
+    the inflow front reaches the cylinder after about 20,000 steps, the wake
+    turns unsteady, and the last half of the run is the measurement window.
+    Sampled every `every` steps: the drag and lift coefficients
+    Fx,y / (u^2 R) and the cross-stream velocity 4 D behind the cylinder.
+    Strouhal number St = D / (u T) from the mean period T between upward
+    zero crossings of that velocity (linearly interpolated); mean drag, lift
+    amplitude.  FP32 and FP16 storage (serving settings)."""
+    import numpy as np
+    from bgk2d_gpu_amd import Solver
+    l, m, D, u = 8192, 2048, 128, 0.05
+    nu = u * D / 200.0
+    xc, yc, r = 1024, m // 2 + 1, D // 2
+    jj, ii = np.mgrid[1:m + 1, 1:l + 1]
+    mask = ((ii - xc) ** 2 + (jj - yc) ** 2 <= r * r).astype(np.int32)
+    del ii, jj
+    box = (xc - r - 3, xc + r + 3, yc - r - 3, yc + r + 3)
+    out = {"workload": f"OPEN channel {l}x{m}, cylinder D={D} at ({xc},{yc}), u={u}, Re=200, from rest, "
+                       f"{steps} steps, sampled every {every}; statistics over the last half",
+           "convective_times": round(steps * u / D, 1)}
+    for prec in ("single", "fp16"):
+        narrow = prec == "fp16"
+        s = Solver(l, m, bc="open", prec=prec, svisc=nu, u0=u, scale=1024.0 if narrow else 1.0,
+                   form=1 if narrow else 0, rounding=1 if narrow else 0)
+        s.set_obstacle(mask)
+        cd, cl, v = [], [], []
+        t0 = time.perf_counter()
+        for k in range(steps // every):
+            s.run(every)
+            fx, fy = s.draglift(box)
+            cd.append(fx / (u * u * r))
+            cl.append(fy / (u * u * r))
+            v.append(s.probe(xc + 4 * D, yc)[1])
+            if (k + 1) % 200 == 0:
+                print(f"# c4 {prec}: step {(k + 1) * every}, Cd {cd[-1]:.4f}, Cl {cl[-1]:.4f}",
+                      file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t0
+        ok = bool(s.check(0.5))
+        s.close()
+        h = len(v) // 2
+        y = np.array(v[h:]) - np.mean(v[h:])
+        up = [i + y[i] / (y[i] - y[i + 1]) for i in range(len(y) - 1) if y[i] < 0 <= y[i + 1]]
+        period = (up[-1] - up[0]) / (len(up) - 1) * every if len(up) >= 3 else None
+        out[prec] = {"mlups_with_sampling": round(l * m * steps / wall / 1e6, 1), "healthy": ok,
+                     "strouhal": round(D / (u * period), 4) if period else None,
+                     "periods_in_window": len(up) - 1,
+                     "cd_mean": round(float(np.mean(cd[h:])), 4),
+                     "cl_amplitude": round(float((np.max(cl[h:]) - np.min(cl[h:])) / 2), 4),
+                     "v_amplitude_over_u": round(float(np.abs(y).max() / u), 4)}
+    a, b = out["single"], out["fp16"]
+    if a["strouhal"] and b["strouhal"]:
+        out["fp16_vs_fp32"] = {"strouhal_rel": round(abs(b["strouhal"] / a["strouhal"] - 1), 5),
+                               "cd_mean_rel": round(abs(b["cd_mean"] / a["cd_mean"] - 1), 5)}
+    return out
+
+
 def copy_calib():
     """Plain device copies of a known size, for scaling the FETCH_SIZE /
     WRITE_SIZE counters to bytes."""
@@ -386,6 +443,9 @@ def main():
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--c4-only", action="store_true")
     ap.add_argument("--c4-steps", type=int, default=2000)
+    ap.add_argument("--c4-shedding", action="store_true",
+                    help="only the C4 channel run long enough to shed (Strouhal number, mean drag)")
+    ap.add_argument("--c4-shedding-steps", type=int, default=400000)
     ap.add_argument("--precision-study", action="store_true",
                     help="only run the FP32/FP16/BF16 vs FP64 Taylor-Green error study")
     ap.add_argument("--study-n", type=int, default=8192)
@@ -412,6 +472,9 @@ def main():
         return
     if args.c4_only:
         print(json.dumps(c4_leg(args.c4_steps)))
+        return
+    if args.c4_shedding:
+        print(json.dumps(c4_shedding(args.c4_shedding_steps)))
         return
     if args.fp16_only:
         print(json.dumps(fp16_leg(args.fp16_steps)))

@@ -955,12 +955,44 @@ __global__ __launch_bounds__(TB_TW) void k_fused_tb_box(Lat L, Phys<C> P, int bx
 
 // ------------------------------------------------------------ diagnostics
 // One block per row j; each thread accumulates a fixed strided subset of
-// the row in FP64 and the block reduces in a fixed tree, so results depend
-// only on (l, m).  Per row: diagno sums (rho, u, v, u2+v2 with diagno's
+// the row in FP64 and the block reduces in a fixed order (wavefront
+// shuffles, then the wavefronts' partials), so results depend only on
+// (l, m).  Per row: diagno sums (rho, u, v, u2+v2 with diagno's
 // per-node formulas), varm sums (rho, u, v with varm's formulas), the max
 // of u2+v2 and a non-finite counter.
 constexpr int ROWS_T = 256;
 constexpr int ROW_VALS = 9;
+
+// Fixed-order block reduction of N FP64 values per thread (value MX by max,
+// the others by sum): each 64-lane wavefront folds its lanes with shuffles
+// (offsets 32, 16, .., 1), lane 0 of each wavefront leaves its N partials in
+// sh[N][ROWS_T / 64], and thread q < N combines value q over the wavefronts
+// in order into acc[0].
+template <int N, int MX>
+__device__ __forceinline__ void block_reduce(double (&acc)[N], double *sh)
+{
+    constexpr int NW = ROWS_T / 64;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int q = 0; q < N; q++) {
+            const double o = __shfl_down(acc[q], off, 64);
+            acc[q] = q == MX ? fmax(acc[q], o) : acc[q] + o;
+        }
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < N; q++) sh[q * NW + w] = acc[q];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < N) {
+        const int q = threadIdx.x;
+        double t = sh[q * NW];
+        for (int k = 1; k < NW; k++) t = q == MX ? fmax(t, sh[q * NW + k]) : t + sh[q * NW + k];
+        acc[0] = t;
+    }
+}
 
 template <class S, class C>
 __global__ __launch_bounds__(ROWS_T) void k_rows(Lat L, Phys<C> P, double *out)
@@ -994,23 +1026,9 @@ __global__ __launch_bounds__(ROWS_T) void k_rows(Lat L, Phys<C> P, double *out)
         acc[7] = sd > acc[7] ? sd : acc[7];
         if (!(sd <= 1.0e300) || !((double)rho <= 1.0e300) || !((double)rho >= -1.0e300)) acc[8] += 1.0;
     }
-    __shared__ double sh[ROW_VALS][ROWS_T];
-#pragma unroll
-    for (int q = 0; q < ROW_VALS; q++) sh[q][threadIdx.x] = acc[q];
-    __syncthreads();
-    for (int w = ROWS_T / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-#pragma unroll
-            for (int q = 0; q < ROW_VALS; q++) {
-                if (q == 7)
-                    sh[q][threadIdx.x] = fmax(sh[q][threadIdx.x], sh[q][threadIdx.x + w]);
-                else
-                    sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
-            }
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x < ROW_VALS) out[(long long)(j - 1) * ROW_VALS + threadIdx.x] = sh[threadIdx.x][0];
+    __shared__ double sh[ROW_VALS][ROWS_T / 64];
+    block_reduce<ROW_VALS, 7>(acc, &sh[0][0]);
+    if (threadIdx.x < ROW_VALS) out[(long long)(j - 1) * ROW_VALS + threadIdx.x] = acc[0];
 }
 
 // sums of the per-row values in a fixed order: out = {Srho, Su, Sv, Ss,
@@ -1028,23 +1046,9 @@ __global__ __launch_bounds__(ROWS_T) void k_rows_final(const double *rows, int m
         acc[4] = fmax(acc[4], r[7]);
         acc[5] += r[8];
     }
-    __shared__ double sh[6][ROWS_T];
-#pragma unroll
-    for (int q = 0; q < 6; q++) sh[q][threadIdx.x] = acc[q];
-    __syncthreads();
-    for (int w = ROWS_T / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-#pragma unroll
-            for (int q = 0; q < 6; q++) {
-                if (q == 4)
-                    sh[q][threadIdx.x] = fmax(sh[q][threadIdx.x], sh[q][threadIdx.x + w]);
-                else
-                    sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
-            }
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x < 6) out[threadIdx.x] = sh[threadIdx.x][0];
+    __shared__ double sh[6][ROWS_T / 64];
+    block_reduce<6, 4>(acc, &sh[0][0]);
+    if (threadIdx.x < 6) out[threadIdx.x] = acc[0];
 }
 
 // prof_i (mode 0, row jc, n = l), prof_j (mode 1, column ic, n = m),
