@@ -61,7 +61,7 @@ def test_batched_members_equal_solves_alone():
 @pytest.mark.gpu
 def test_two_step_batch_members_equal_solves_alone():
     """groups large enough to fill the chip advance two steps per launch
-    (fp32 by default, 16-bit kinds with tblock = 2), their last partial
+    (every storage kind by default), their last partial
     chunk one step per launch: each result is bit-identical to the same
     request run alone"""
     reqs = [Request(i, "cavity", 1024, 1000.0, 64 + 3 * i, "single", 0.0) for i in range(4)]

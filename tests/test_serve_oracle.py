@@ -88,7 +88,7 @@ def test_one_step_groups_equal_oracle():
 
 def test_two_step_groups_equal_oracle():
     """groups that fill the chip with strips of >= 8 rows advance two
-    steps per launch (FP32 by default, FP16 with tblock = 2); the odd
+    steps per launch (every storage kind by default); the odd
     remainder one step per launch"""
     members = [(1024, 800, "driven", "single", 0.05, 0.1, 40 + 2 * i, 0.1, 0, 0) for i in range(4)]
     members += [(1024, 800, "periodic", "fp16", 0.05, 0.0, 30 + i, 0.01, 0, 2) for i in range(4)]
