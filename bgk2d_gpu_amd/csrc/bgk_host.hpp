@@ -349,11 +349,13 @@ int launch_step(bgk_ctx *c)
 // where it beats one step per pass (Taylor-Green 8192^2: FP32 122 k against
 // 75 k MLUPS, FP16 157 k against 145 k).  Needs size_tb first.
 constexpr int TB_HMIN = 8;
-// Blocks per CU that the two-step grids are sized for, at most: the 16-bit
-// instances that the occupancy query allows seven of ran slower sized for
-// seven than for six (Taylor-Green 8192^2 FP16: 158 k against 161 k MLUPS),
-// and six is what the fp32 instances hold anyway (their LDS ring).
-constexpr int TB_MAX_PER_CU = 6;
+// Wavefronts per CU that the two-step grids are sized for, at most: 16-bit
+// 256-column blocks, which the occupancy query allows seven of, ran slower
+// sized for seven than for six (Taylor-Green 8192^2 FP16: 158 k against
+// 161 k MLUPS), and six is what the fp32 blocks hold anyway (their LDS
+// ring).
+constexpr int TB_MAX_WAVES_PER_CU = 24;
+constexpr int TB_MAX_PER_CU = TB_MAX_WAVES_PER_CU / (TB_TW / 64);  // batch blocks
 inline bool use_tb(const bgk_ctx *c)
 {
     if (!foldable(c)) return false;
@@ -370,11 +372,11 @@ inline bool use_tb(const bgk_ctx *c)
 // edge (the obstacle touches the second or second-to-last row or column),
 // every block along that direction is taken.
 // (first strip, last strip, first row, last row) of blocks; x > y: none.
-inline int4 tb_box(const bgk_ctx *c, int H, bool xper, bool yper)
+inline int4 tb_box(const bgk_ctx *c, int H, int TW, bool xper, bool yper)
 {
     if (!(c->p.obstacle && c->nobs > 0)) return make_int4(0, -1, 0, -1);
-    const int strips = (c->l + TB_TW - 1) / TB_TW, rows = (c->m + H - 1) / H;
-    int4 b = make_int4(max(0, (c->imin - 3) / TB_TW - 1), min(strips - 1, (c->imax + 1) / TB_TW + 1),
+    const int strips = (c->l + TW - 1) / TW, rows = (c->m + H - 1) / H;
+    int4 b = make_int4(max(0, (c->imin - 3) / TW - 1), min(strips - 1, (c->imax + 1) / TW + 1),
                        max(0, (c->jmin - 3) / H - 1), min(rows - 1, (c->jmax + 1) / H + 1));
     if (xper && (c->imin - 1 <= 1 || c->imax + 1 >= c->l)) {
         b.x = 0;
@@ -394,6 +396,7 @@ inline int4 tb_box(const bgk_ctx *c, int H, bool xper, bool yper)
 template <class S, class C, int BC, bool OBS>
 int size_tb(bgk_ctx *c)
 {
+    constexpr int TW = tb_tw<S>();
     int per_cu = 0, cus = 0;
     // with an obstacle the grid is still that of the obstacle-free kernel:
     // the obstacle's few strip blocks run beside it (launch_tb_t); leaving
@@ -401,15 +404,15 @@ int size_tb(bgk_ctx *c)
     hipError_t e = hipSuccess;
     with_fl<S>(c, [&](auto flc) {
         constexpr int FL = decltype(flc)::value;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused_tb<S, C, BC, false, FL>, TB_TW, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused_tb<S, C, BC, false, FL>, TW, 0);
     });
     if (e != hipSuccess) return fail(c, BGK_E_HIP, "hipOccupancyMaxActiveBlocksPerMultiprocessor: %s", hipGetErrorString(e));
     HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->dev));
-    if (per_cu > TB_MAX_PER_CU) per_cu = TB_MAX_PER_CU;
+    if (per_cu > TB_MAX_WAVES_PER_CU / (TW / 64)) per_cu = TB_MAX_WAVES_PER_CU / (TW / 64);
     if (per_cu < 1) per_cu = 1;
     if (cus < 1) cus = 1;
     const long long slots = (long long)per_cu * cus;
-    const long long strips = (c->l + TB_TW - 1) / TB_TW;
+    const long long strips = (c->l + TW - 1) / TW;
     long long per_strip = slots / strips;
     if (per_strip < 1) per_strip = 1;
     const int h = (int)((c->m + per_strip - 1) / per_strip);
@@ -438,13 +441,14 @@ int launch_tb_t(bgk_ctx *c)
         int r = size_tb<S, C, BC, OBS>(c);
         if (r != BGK_OK) return r;
     }
+    constexpr int TW = tb_tw<S>();
     Lat L = lat(c);
     Phys<C> P = phys<C>(c);
     const int H = c->tb_h;
-    const int strips = (c->l + TB_TW - 1) / TB_TW, rows = (c->m + H - 1) / H;
+    const int strips = (c->l + TW - 1) / TW, rows = (c->m + H - 1) / H;
     const int rows8 = (rows + NXCD - 1) / NXCD * NXCD;
     // strip blocks with the obstacle rule
-    const int4 box = OBS ? tb_box(c, H, Per<BC>::x, Per<BC>::y) : make_int4(0, -1, 0, -1);
+    const int4 box = OBS ? tb_box(c, H, TW, Per<BC>::x, Per<BC>::y) : make_int4(0, -1, 0, -1);
     if (OBS && box.y >= box.x) {
         // the obstacle's blocks on a second stream, beside the others:
         // the rows of the skipped blocks, in blocks of a quarter of H
@@ -454,14 +458,14 @@ int launch_tb_t(bgk_ctx *c)
         HIPCHK(c, hipStreamWaitEvent(c->s2, c->evf, 0));
         with_fl<S>(c, [&](auto flc) {
             constexpr int FL = decltype(flc)::value;
-            hipLaunchKernelGGL((k_fused_tb_box<S, C, BC, FL>), dim3(nbx * nsub), dim3(TB_TW), 0, c->s2, L, P, box.x,
+            hipLaunchKernelGGL((k_fused_tb_box<S, C, BC, FL>), dim3(nbx * nsub), dim3(TW), 0, c->s2, L, P, box.x,
                                nbx, r0, r1, hb);
         });
         HIPCHK(c, hipEventRecord(c->evj, c->s2));
     }
     with_fl<S>(c, [&](auto flc) {
         constexpr int FL = decltype(flc)::value;
-        hipLaunchKernelGGL((k_fused_tb<S, C, BC, false, FL>), dim3(strips * rows8), dim3(TB_TW), 0, c->s, L, P, H,
+        hipLaunchKernelGGL((k_fused_tb<S, C, BC, false, FL>), dim3(strips * rows8), dim3(TW), 0, c->s, L, P, H,
                            strips, rows, box);
     });
     if (OBS && box.y >= box.x) HIPCHK(c, hipStreamWaitEvent(c->s, c->evj, 0));

@@ -758,11 +758,20 @@ __global__ __launch_bounds__(256) void k_fused2(Lat L, Phys<C> P)
 // r+1 overwrites ring row r-3, read last by phase 2 of row r-2, which every
 // thread has finished when it passes the barrier of row r.
 // strip width of the two-step kernels: one column per thread, one block
-// per strip
+// per strip.  TB_TW is the width of the batch kernel and of bgk_run's fp32
+// and fp64 kernels; bgk_run's 16-bit kernels use strips twice as wide (FP16
+// Taylor-Green 8192^2: 165 k against 161 k MLUPS; for fp32 512 columns gave
+// 128.5 k against 128.2 k, and the fp64 ring would not fit; 64, 128 and
+// 1024 columns were slower for every kind).
 #ifndef BGK_TB_TW
 #define BGK_TB_TW 256
 #endif
 constexpr int TB_TW = BGK_TB_TW;
+template <class S>
+__host__ __device__ constexpr int tb_tw()
+{
+    return sizeof(S) == 2 ? 2 * TB_TW : TB_TW;
+}
 constexpr int TB_RB = 16;   // rows of rim columns computed per batch (power of two; 16 beat 8, 32, 64)
 
 // ring slot of population k (0..7, not 08/17): 01,03,05,10,12,14 -> 0..5
@@ -807,12 +816,13 @@ __device__ __forceinline__ void tb_load(const Lat &L, int i, int j, R xr[9])
     xr[8] = *(const R *)(pull_base<S>(L, L.r, 8) + bo);
 }
 
-// the strip block of strip bx (columns bx*TB_TW+1..) and rows j0..jend
-template <class S, class C, int BC, bool OBS, int FL>
+// the strip block of strip bx (columns bx*TW+1..) and rows j0..jend, run
+// by TW threads
+template <class S, class C, int BC, bool OBS, int FL, int TW = TB_TW>
 __device__ __forceinline__ void tb_block(const Lat &L, const Phys<C> &P, int bx, int j0, int jend)
 {
     using R = typename CodecT<S, C>::raw;
-    constexpr int TW = TB_TW, E = TW + 2;
+    constexpr int E = TW + 2;
     constexpr bool XP = Per<BC>::x, YP = Per<BC>::y;
     __shared__ R ring[4][6][E];
     constexpr int RB = TB_RB;
@@ -845,7 +855,7 @@ __device__ __forceinline__ void tb_block(const Lat &L, const Phys<C> &P, int bx,
             // 10,12,14.  The previous segment's rim rows were all copied
             // into the ring before the barrier of their own row.
             const int nr = min(RB, jend + 2 - r);
-            for (int t = tid; t < 2 * nr; t += TB_TW) {
+            for (int t = tid; t < 2 * nr; t += TW) {
                 const int side = t >= nr ? 1 : 0, rr = t - side * nr;
                 const int wi = wcol(side ? i0 + ncols : i0 - 1), wj = wrow(r + rr);
                 if (wi < 0 || wj < 0) continue;
@@ -931,12 +941,12 @@ __device__ __forceinline__ bool tb_xcd_block(int b, int strips, int rows, int &b
 // box = (first, last strip; first, last row of strip blocks): blocks left
 // to k_fused_tb_box (an empty range, x > y, skips none)
 template <class S, class C, int BC, bool OBS, int FL = -1>
-__global__ __launch_bounds__(TB_TW) void k_fused_tb(Lat L, Phys<C> P, int H, int strips, int rows, int4 box)
+__global__ __launch_bounds__(tb_tw<S>()) void k_fused_tb(Lat L, Phys<C> P, int H, int strips, int rows, int4 box)
 {
     int bx, by;
     if (!tb_xcd_block((int)blockIdx.x, strips, rows, bx, by)) return;
     if (bx >= box.x && bx <= box.y && by >= box.z && by <= box.w) return;
-    tb_block<S, C, BC, OBS, FL>(L, fixed_flags<S, C, BC, FL>(P), bx, by * H + 1, min(by * H + H, L.m));
+    tb_block<S, C, BC, OBS, FL, tb_tw<S>()>(L, fixed_flags<S, C, BC, FL>(P), bx, by * H + 1, min(by * H + H, L.m));
 }
 
 // The strip blocks that reach the obstacle, with the obstacle rule: strips
@@ -946,11 +956,11 @@ __global__ __launch_bounds__(TB_TW) void k_fused_tb(Lat L, Phys<C> P, int H, int
 // carry it, shorter than the others so that they end with them, and the
 // grid of the others keeps its occupancy.
 template <class S, class C, int BC, int FL = -1>
-__global__ __launch_bounds__(TB_TW) void k_fused_tb_box(Lat L, Phys<C> P, int bx0, int nbx, int r0, int r1, int hb)
+__global__ __launch_bounds__(tb_tw<S>()) void k_fused_tb_box(Lat L, Phys<C> P, int bx0, int nbx, int r0, int r1, int hb)
 {
     const int j0 = r0 + ((int)blockIdx.x / nbx) * hb;
-    tb_block<S, C, BC, true, FL>(L, fixed_flags<S, C, BC, FL>(P), bx0 + (int)blockIdx.x % nbx, j0,
-                                 min(j0 + hb - 1, r1));
+    tb_block<S, C, BC, true, FL, tb_tw<S>()>(L, fixed_flags<S, C, BC, FL>(P), bx0 + (int)blockIdx.x % nbx, j0,
+                                             min(j0 + hb - 1, r1));
 }
 
 // ------------------------------------------------------------ diagnostics
