@@ -423,13 +423,15 @@ def test_half_storage_tgv_1024_matches_reference_mixed1_build():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["single", "fp16"])
 @pytest.mark.parametrize("bc", ["periodic", "channel", "inflow"])
 @pytest.mark.parametrize("edge", ["bottom", "top", "left", "right"])
-def test_two_step_obstacle_at_periodic_edges(bc, edge):
+def test_two_step_obstacle_at_periodic_edges(bc, edge, prec):
     """a small obstacle on the second or second-to-last row or column: along
     a periodic direction the strip blocks at the far edge update wrapped
     nodes next to it, so they too apply the obstacle rule; the result
-    equals the one-step kernel bit for bit"""
+    equals the one-step kernel bit for bit (fp32: 256-column strips,
+    16-bit kinds: 512)"""
     l, m = 600, 260
     mask = np.zeros((m, l), dtype=np.int32)
     if edge == "bottom":
@@ -441,12 +443,15 @@ def test_two_step_obstacle_at_periodic_edges(bc, edge):
     else:
         mask[40:46, l - 4:l - 1] = 1  # columns l-3..l-1
     out = []
+    narrow = prec == "fp16"
     for tb in (1, 2):
-        print(f"case {bc} {edge} tblock={tb}", flush=True)
-        s = Solver(l, m, bc=bc, impl="fused", prec="single", svisc=0.05, u0=0.05, tblock=tb)
+        print(f"case {bc} {edge} {prec} tblock={tb}", flush=True)
+        s = Solver(l, m, bc=bc, impl="fused", prec=prec, svisc=0.05, u0=0.05, tblock=tb,
+                   scale=256.0 if narrow else 1.0, form=int(narrow), rounding=int(narrow))
         s.set_obstacle(mask)
         s.run(41)
         out.append(s.raw())
         s.close()
     a, b = out
-    assert np.array_equal(a[:, 1:-1, 1:-1].view(np.uint32), b[:, 1:-1, 1:-1].view(np.uint32))
+    u = np.uint16 if narrow else np.uint32
+    assert np.array_equal(a[:, 1:-1, 1:-1].view(u), b[:, 1:-1, 1:-1].view(u))
