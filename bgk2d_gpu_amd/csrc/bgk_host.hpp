@@ -349,13 +349,14 @@ int launch_step(bgk_ctx *c)
 // where it beats one step per pass (Taylor-Green 8192^2: FP32 122 k against
 // 75 k MLUPS, FP16 157 k against 145 k).  Needs size_tb first.
 constexpr int TB_HMIN = 8;
-// Wavefronts per CU that the two-step grids are sized for, at most: 16-bit
-// 256-column blocks, which the occupancy query allows seven of, ran slower
+// Wavefronts per CU that the two-step grids are sized for, at most (a
+// grid is one round of the blocks the device holds): 16-bit blocks of 256
+// columns, which the occupancy query allows seven of per CU, ran slower
 // sized for seven than for six (Taylor-Green 8192^2 FP16: 158 k against
-// 161 k MLUPS), and six is what the fp32 blocks hold anyway (their LDS
-// ring).
+// 161 k MLUPS); the fp32 blocks hold six anyway (their LDS ring), the
+// 16-bit 512-column blocks three (their registers).
 constexpr int TB_MAX_WAVES_PER_CU = 24;
-constexpr int TB_MAX_PER_CU = TB_MAX_WAVES_PER_CU / (TB_TW / 64);  // batch blocks
+constexpr int TB_MAX_PER_CU = TB_MAX_WAVES_PER_CU / (TB_TW / 64);  // blocks of the batch kernel
 inline bool use_tb(const bgk_ctx *c)
 {
     if (!foldable(c)) return false;
@@ -927,9 +928,9 @@ inline bool member_solo(const bgk_ctx *c) { return (size_t)c->l * (size_t)c->m >
 
 // A member advances two steps per launch (k_batch_tb) in a chunk of even
 // length that it runs to the end, with bgk_run's rule for the kernel:
-// tblock = 0 or 2, every storage kind (16-bit members in two-step groups:
-// 111.8 k against 108.5 k MLUPS on the full serving stream than in one-step
-// groups); tblock = 1 never; otherwise one step per launch.
+// tblock 0 or 2, every storage kind; tblock 1 never; otherwise one step per
+// launch.  (The full serving stream with its 16-bit members in two-step
+// groups: 111.8 k MLUPS; in one-step groups: 108.5 k.)
 inline bool member_tb(const bgk_ctx *c, long long remaining, int chunk)
 {
     const bool two = c->p.tblock != 1;
