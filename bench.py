@@ -414,6 +414,31 @@ def copy_calib():
     print(json.dumps({"copy_calib_bytes_each_way": a.numel() * 4, "copies": 5}))
 
 
+def copy_rate():
+    """Measured device-copy rate on this box (GB/s, read + write bytes): a
+    plain 1 GiB copy, best of 10, timed with events on torch's stream (the
+    stream the copy runs on)"""
+    import torch
+    a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    best = None
+    for _ in range(10):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None or ms < best else best
+    nbytes = 2 * a.numel() * 4
+    del a, b
+    torch.cuda.empty_cache()
+    return round(nbytes / (best / 1e3) / 1e9, 1)
+
+
 def profiled_traffic(name):
     """HBM bytes per launch of a kernel measured with FETCH_SIZE and
     WRITE_SIZE (calibrated on a 1 GiB copy), from its committed summary"""
@@ -533,6 +558,7 @@ def main():
                      "traffic_source": "profiles/k_fused_tb_fp32_ldc4096_hbm_counters.md"},
         "mlups_over_one_step_roofline": round(mlups * 1e6 * BYTES_PER_LUP["single"] / 1e9
                                               / HBM_PEAK_GBS, 4),
+        "measured_copy_gbs": copy_rate(),
         "one_step_per_pass": one,
         "kernel_ms_per_step": r32["kernel_ms"],
         "healthy": bool(r32["ok"] and r1["ok"]),
