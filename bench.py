@@ -417,7 +417,8 @@ def copy_calib():
 def copy_rate():
     """Measured device-copy rate on this box (GB/s, read + write bytes): a
     plain 1 GiB copy, best of 10, timed with events on torch's stream (the
-    stream the copy runs on)"""
+    stream the copy runs on).  Run in a process of its own (--copy-rate):
+    torch's HIP runtime and the library's do not share a process."""
     import torch
     a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
@@ -437,6 +438,16 @@ def copy_rate():
     del a, b
     torch.cuda.empty_cache()
     return round(nbytes / (best / 1e3) / 1e9, 1)
+
+
+def copy_rate_child():
+    """copy_rate() in a child process, before this one touches the GPU"""
+    try:
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-rate"], stdout=subprocess.PIPE,
+                           stderr=subprocess.DEVNULL, text=True, timeout=300)
+        return json.loads(r.stdout.strip().splitlines()[-1])["copy_gbs"]
+    except (OSError, subprocess.SubprocessError, ValueError, KeyError, IndexError):
+        return None
 
 
 def profiled_traffic(name):
@@ -478,7 +489,11 @@ def main():
     ap.add_argument("--study-init", type=int, default=0)
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run 5 torch copies of 1 GiB (HBM counter calibration)")
+    ap.add_argument("--copy-rate", action="store_true", help="only the measured device-copy rate")
     args = ap.parse_args()
+    if args.copy_rate:
+        print(json.dumps({"copy_gbs": copy_rate()}))
+        return
     if args.copy_calib:
         copy_calib()
         return
@@ -508,6 +523,7 @@ def main():
         print(json.dumps(cpu_baseline()))
         return
 
+    copy_gbs = copy_rate_child()
     n = 4096
     # the FP32 solve as bgk_run does it: two FUSED steps per pass over HBM
     r32 = gpu_solve(n, "single", "driven", args.steps, args.warmup, 0.04096, 0.1, tblock=2)
@@ -558,7 +574,7 @@ def main():
                      "traffic_source": "profiles/k_fused_tb_fp32_ldc4096_hbm_counters.md"},
         "mlups_over_one_step_roofline": round(mlups * 1e6 * BYTES_PER_LUP["single"] / 1e9
                                               / HBM_PEAK_GBS, 4),
-        "measured_copy_gbs": copy_rate(),
+        "measured_copy_gbs": copy_gbs,
         "one_step_per_pass": one,
         "kernel_ms_per_step": r32["kernel_ms"],
         "healthy": bool(r32["ok"] and r1["ok"]),
