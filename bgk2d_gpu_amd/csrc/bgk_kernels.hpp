@@ -783,6 +783,21 @@ __host__ __device__ constexpr int tb_tw()
 }
 constexpr int TB_RB = 16;   // rows of rim columns computed per batch (power of two; 16 beat 8, 32, 64)
 
+// Two-step instances that march with no branch around their loads and
+// stores (tb_block), under a bound of six wavefronts per SIMD (80 VGPRs):
+// 16-bit storage with the boundary variants where that measured faster.
+// MLUPS, branched march -> branch-free (FP16, 8192^2 Taylor-Green,
+// 4096^2 cavity, 8192x2048 channels): periodic 165 k -> 186 k, driven
+// 152 k -> 171 k, channel 141 k -> 156 k, inflow 91 k -> 110 k; open
+// (120 k -> 104 k) and couette (127 k -> 113 k) lost occupancy to the
+// extra registers and keep the branched march, as do fp32 (127.5 k ->
+// 127.0 k) and fp64.
+template <class S, int BC>
+__host__ __device__ constexpr bool tb_branch_free()
+{
+    return sizeof(S) == 2 && (BC == BK_DRIVEN || BC == BK_PERIODIC || BC == BK_CHANNEL || BC == BK_INFLOW);
+}
+
 // ring slot of population k (0..7, not 08/17): 01,03,05,10,12,14 -> 0..5
 __device__ __forceinline__ constexpr int tb_slot(int k) { return k < 3 ? k : k - 1; }
 
@@ -926,16 +941,14 @@ __device__ __forceinline__ void tb_block(const Lat &L, const Phys<C> &P, int bx,
         for (int k = 0; k < 8; k++) st_nt((R *)(pull_base<S>(L, L.b[k], 8) + bo), encode_s<S, C>(P, o[k], salt, k));
         st_nt((R *)(pull_base<S>(L, L.rn, 8) + bo), encode_s<S, C>(P, o[8], salt, 8));
     };
-    // 16-bit kinds: the same march with no branch around its loads or
-    // stores.  Every thread pulls each row (those past the strip from
-    // column 1) and stores each row (those past the strip into one slot of
-    // the destination set's bottom halo row, which no folded step reads:
-    // edge pulls from it are replaced).
-    // vmcnt counts loads and stores in issue order, so the wait for a row's
-    // pulls then leaves the previous row's stores in flight; behind a
-    // branch that may skip them, the compiler's wait covers those stores
-    // too (FP16 Taylor-Green 8192^2: 182.6 k against 164.9 k MLUPS; fp32
-    // unchanged, 127.0 k against 127.5 k, so it keeps the march above).
+    // The same march with no branch around its loads or stores
+    // (tb_branch_free): every thread pulls each row (those past the strip
+    // from column 1) and stores each row (those past the strip into one
+    // slot of the destination set's bottom halo row, which no folded step
+    // reads: edge pulls from it are replaced).  vmcnt counts loads and
+    // stores in issue order, so the wait for a row's pulls then leaves the
+    // previous row's stores in flight; behind a branch that may skip them,
+    // the compiler's wait covers those stores too.
     auto safe = [&](int r) { return r < 1 ? 1 : (r > m ? m : r); };
     const int li = own ? i : 1;
     // one slot per block (the lanes past the strip coalesce into one write)
@@ -1013,7 +1026,7 @@ __device__ __forceinline__ void tb_block(const Lat &L, const Phys<C> &P, int bx,
     using NoP2 = std::integral_constant<bool, false>;
     using P2 = std::integral_constant<bool, true>;
     R xa[9];
-    if constexpr (sizeof(S) == 2) {
+    if constexpr (tb_branch_free<S, BC>()) {
         (void)row;
         {
             const int w0 = wrow(j0 - 1);
@@ -1049,7 +1062,8 @@ __device__ __forceinline__ bool tb_xcd_block(int b, int strips, int rows, int &b
 // box = (first, last strip; first, last row of strip blocks): blocks left
 // to k_fused_tb_box (an empty range, x > y, skips none)
 template <class S, class C, int BC, bool OBS, int FL = -1>
-__global__ __launch_bounds__(tb_tw<S>()) void k_fused_tb(Lat L, Phys<C> P, int H, int strips, int rows, int4 box)
+__global__ __launch_bounds__(tb_tw<S>(), (tb_branch_free<S, BC>() ? 6 : 1)) void k_fused_tb(Lat L, Phys<C> P, int H,
+                                                                                           int strips, int rows, int4 box)
 {
     int bx, by;
     if (!tb_xcd_block((int)blockIdx.x, strips, rows, bx, by)) return;
