@@ -187,7 +187,10 @@ int bgk_collision(bgk_ctx_t *ctx, int32_t itime);
  * runs the boundary folded into the pull, one or two steps per kernel
  * (params.tblock); the rest replay boundaries/propagation/collision.
  * Steps are captured into HIP graphs in chunks.  Interior results are
- * identical to the per-phase entry points.                               */
+ * identical to the per-phase entry points; the folded path neither reads
+ * nor maintains the halo ring (the two-step 16-bit kernels may leave
+ * arbitrary values in the bottom halo row), which bgk_boundaries rewrites
+ * before any per-phase step reads it.                                    */
 int bgk_run(bgk_ctx_t *ctx, int32_t nsteps);
 int bgk_sync(bgk_ctx_t *ctx);
 
