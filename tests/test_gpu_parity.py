@@ -396,6 +396,20 @@ def test_c2_cavity_4096_fused_matches_reference_builds():
 
 
 @pytest.mark.gpu
+def test_c2_cavity_4096_full_length_matches_reference_builds():
+    """C2 at its full length: lid-driven cavity 4096^2, FUSED, Re = 1e4,
+    20,000 steps through bgk_run: the interior populations carry the
+    SHA-256 of the reference's own SINGLE and DOUBLE builds after 20,000
+    steps (tests/golden/make_cavity_reference.py c2, about four hours of
+    CPU for the two builds); FP32 against FP64 as recorded there"""
+    ref, pops = _reference_digest_case("c2", "driven", 4096, 4096, 0.04096, 0.1, 20000,
+                                       ("single", "double"))
+    linf, rel = _uv_diff(pops["single"], pops["double"])
+    assert linf == pytest.approx(ref["single_vs_double"]["uv_linf"], rel=1e-9)
+    assert rel == pytest.approx(ref["single_vs_double"]["uv_rel_l2"], rel=1e-9)
+
+
+@pytest.mark.gpu
 def test_c3_tgv_8192_fused_matches_reference_build():
     """C3 size: Taylor-Green 8192^2, FUSED, tau = 0.8 (svisc = 0.1), the
     reference's initial field (amplitude 0.1), 5,000 steps through bgk_run:
