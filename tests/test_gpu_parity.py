@@ -161,6 +161,10 @@ OPTION_CASES = [
     ("couette", 0, "single", 1, 0, 0),
     ("periodic", 0, "double", 1, 0, 1),
     ("step", 0, "fp16", 1, 1, 0),
+    ("open", 1, "fp16", 1, 1, 0),
+    ("open", 0, "bf16", 1, 1, 0),
+    ("inflow", 0, "bf16", 1, 1, 0),
+    ("couette", 0, "fp16", 1, 1, 0),
 ]
 
 
