@@ -490,6 +490,8 @@ def main():
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run 5 torch copies of 1 GiB (HBM counter calibration)")
     ap.add_argument("--copy-rate", action="store_true", help="only the measured device-copy rate")
+    ap.add_argument("--no-copy-rate", action="store_true",
+                    help="skip the device-copy child process (profiled runs: it is a second GPU process)")
     args = ap.parse_args()
     if args.copy_rate:
         print(json.dumps({"copy_gbs": copy_rate()}))
@@ -523,7 +525,7 @@ def main():
         print(json.dumps(cpu_baseline()))
         return
 
-    copy_gbs = copy_rate_child()
+    copy_gbs = None if args.no_copy_rate else copy_rate_child()
     n = 4096
     # the FP32 solve as bgk_run does it: two FUSED steps per pass over HBM
     r32 = gpu_solve(n, "single", "driven", args.steps, args.warmup, 0.04096, 0.1, tblock=2)
