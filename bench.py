@@ -126,7 +126,7 @@ def cpu_baseline():
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_SINGLE_FUSED_LDC")
     host = host_info()
     if os.path.exists(exe):
-        serial = run_reference(exe, n, 30)
+        serial = run_reference(exe, n, 150)  # about 10 s of one core
         if serial:
             out = {"value": serial["mlups"], "unit": "MLUPS", "cores": 1, "kind": "reference",
                    "sample": f"4096x4096 LDC Re=1e4, {serial['steps']} steps, serial flang -O3 build "
