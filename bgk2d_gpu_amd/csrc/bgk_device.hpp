@@ -222,7 +222,7 @@ struct Phys {
     // kernels that compute on scaled populations (scale*(f - w) as stored,
     // see fixed_flags): the storage scale and its inverse; 1 otherwise
     C usc, dsc;
-    int noshift;  // 1: whole populations f (the NOSHIFT build), per-phase kernels only
+    int noshift;  // 1: whole populations f (the NOSHIFT build)
 };
 
 // ---------------------------------------------------------- arithmetic

@@ -118,9 +118,11 @@ void derive(bgk_ctx *c)
     const C cvsq = xj * xj + yj * yj;
     const C cxq[3] = {eq_d<C>(xj - yj, cvsq), eq_d<C>(xj + yj, cvsq), eq_d<C>(xj, cvsq)};
     const C wq[3] = {K<C>::p2, K<C>::p2, K<C>::p1};
-    // crho*p*(cte1+cx) in the compute kind; each stored copy is rounded to
-    // the storage kind where it is stored (x = 0, row j)
-    for (int q = 0; q < 3; q++) c->inflow[q] = (double)((C)1.0 * wq[q] * ((C)0.0 + cxq[q]));
+    // crho*p*(cte1+cx) in the compute kind (cte1 = 1 for whole populations,
+    // SRC/bcond.inflow.F90:48-52); each stored copy is rounded to the
+    // storage kind where it is stored (x = 0, row j)
+    const C cte1 = p.noshift ? (C)1.0 : (C)0.0;
+    for (int q = 0; q < 3; q++) c->inflow[q] = (double)((C)1.0 * wq[q] * (cte1 + cxq[q]));
 }
 
 template <class C>
@@ -156,11 +158,13 @@ Phys<C> phys(const bgk_ctx *c)
 
 // Kernel instance flags (Cfg in bgk_kernels.hpp): 0 for the reference's
 // form with nearest-even stores, 1 for the deviation form with stochastic
-// rounding of 16-bit stores, -1 (16-bit kinds only) for the other mixes.
+// rounding of 16-bit stores, 2 (fp32/fp64 storage) for whole populations
+// (NOSHIFT, which needs form 0), -1 (16-bit kinds only) for the other mixes.
 inline int fl_of(const bgk_ctx *c)
 {
     const bool narrow = is_fp16(c->p.prec) || is_bf16(c->p.prec);
     const int form = c->p.form, rnd = narrow ? c->p.rounding : form;
+    if (c->p.noshift) return narrow ? -1 : 2;
     if (form == 0 && rnd == 0) return 0;
     if (form == 1 && rnd == 1) return 1;
     return -1;
@@ -176,6 +180,8 @@ void with_fl(const bgk_ctx *c, F &&f)
         f(std::integral_constant<int, 1>{});
     } else if (fl == 0) {
         f(std::integral_constant<int, 0>{});
+    } else if (fl == 2) {
+        if constexpr (sizeof(S) != 2) f(std::integral_constant<int, 2>{});
     } else {
         if constexpr (sizeof(S) == 2) f(std::integral_constant<int, -1>{});
     }
@@ -291,11 +297,10 @@ int launch_collision(bgk_ctx *c)
     return BGK_OK;
 }
 
-// the boundary folded into the hot kernels: FUSED, all variants but STEP,
-// on deviations f - w (NOSHIFT takes the per-phase kernels)
+// the boundary folded into the hot kernels: FUSED, all variants but STEP
 inline bool foldable(const bgk_ctx *c)
 {
-    return c->p.impl == BGK_IMPL_FUSED && c->p.bc != BGK_BC_STEP && !c->p.noshift;
+    return c->p.impl == BGK_IMPL_FUSED && c->p.bc != BGK_BC_STEP;
 }
 
 template <class S, class C, int BC, bool OBS>

@@ -485,9 +485,10 @@ __device__ C folded_a(const A &a, const Phys<C> &P, int l, int m, int k, int i, 
         const int sjw = sj < 1 ? m : (sj > m ? 1 : sj);  // the y pass runs after the x pass
         if (si < 1) return rt<S, C>(P, P.inflow[k], 0, sjw, k);  // pops 01,03,05 from x = 0
         if (si > l) {
-            // outflow equilibrium from the momentum of (l, sj)
+            // outflow equilibrium from the momentum of (l, sj); whole
+            // populations in the NOSHIFT build (SRC/bcond.inflow.F90:48-52)
             const C rf = K<C>::rf, qf = K<C>::qf, p1 = K<C>::p1, p2 = K<C>::p2;
-            const C rhoinv = (C)1.0, crho = (C)1.0, cte1 = (C)0.0;
+            const C rhoinv = (C)1.0, crho = (C)1.0, cte1 = P.noshift ? (C)1.0 : (C)0.0;
             C x01 = a(0, l, sj), x03 = a(1, l, sj);
             C x05 = a(2, l, sj), x08 = a(3, l, sj);
             C x10 = a(4, l, sj), x12 = a(5, l, sj);
@@ -582,9 +583,10 @@ __device__ __forceinline__ void fix_pulls(const Lat &L, const Phys<C> &P, int i,
 }
 
 // Compile-time configuration of a hot-kernel instance.  FL: -1 the generic
-// instance (form and rounding read from Phys); 0 the reference's form with
-// nearest-even stores; 1 the deviation form with stochastic rounding of
-// 16-bit stores.  The host picks 0 or 1 when the parameters are those, so
+// instance (form, rounding and the NOSHIFT switch read from Phys); 0 the
+// reference's form with nearest-even stores; 1 the deviation form with
+// stochastic rounding of 16-bit stores; 2 the reference's form on whole
+// populations (the NOSHIFT build, SRC/col_MC.F90:70-74).  The host picks 0 or 1 when the parameters are those, so
 // the values are those of the generic instance; fixing them lets the
 // compiler drop the other form's density terms and store path.  With FL = 1
 // and 16-bit storage, the variants whose folded boundary rules are copies
@@ -603,8 +605,9 @@ __device__ __forceinline__ Phys<C> fixed_flags(const Phys<C> &P)
 {
     Phys<C> Q = P;
     if constexpr (FL >= 0) {
-        Q.form = FL;
-        Q.rounding = FL;  // only the 16-bit codecs read it
+        Q.form = FL == 1 ? 1 : 0;
+        Q.rounding = FL == 1 ? 1 : 0;  // only the 16-bit codecs read it
+        Q.noshift = FL == 2 ? 1 : 0;
     }
     if constexpr (Cfg<S, C, BC, FL>::SCALED) {
         // populations stay scale*(f - w) in registers: codecs without the
@@ -622,7 +625,7 @@ template <class S, class C, int BC, int FL>
 __device__ __forceinline__ void relax_k(const Phys<C> &P, const C x[9], C o[9])
 {
     using K = Cfg<S, C, BC, FL>;
-    relax<C, K::FORCE, FL, K::SCALED, K::Z0>(P, x, o);
+    relax<C, K::FORCE, (FL == 2 ? 0 : FL), K::SCALED, K::Z0, (FL == 2 || FL < 0)>(P, x, o);
 }
 
 // Addressing of the hot kernel: one 32-bit byte offset per node, and per

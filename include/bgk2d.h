@@ -116,8 +116,10 @@ typedef struct bgk_params {
     int32_t noshift;      /* 1: whole populations f stored instead of the
                              deviations f - w, the NOSHIFT build
                              (SRC/col_MC.F90:70-74, SRC/init.F90:73-77);
-                             form 0 only, not OPEN; bgk_run takes the
-                             per-phase kernels                              */
+                             form 0 only, not OPEN; bgk_run folds the
+                             boundary into the one- and two-step kernels
+                             as for shifted populations (SRC/bcond.inflow
+                             .F90:48-52 for the inflow equilibria)          */
     double svisc;         /* namelist svisc (kinematic viscosity)           */
     double u0;            /* namelist u0 (lid speed, or channel force scale)*/
     double scale;         /* FP16/BF16 storage scale (power of two)         */

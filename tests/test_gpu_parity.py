@@ -39,7 +39,7 @@ def same_bits(a, b, prec):
     return (a.view(u) == b.view(u)) | nan
 
 
-def fixture_solver(name, impl_override=None):
+def fixture_solver(name, impl_override=None, **kw):
     p, impl, case = name.split("@")[0].split("_")
     lx, ly, ns, rad = (int(x) for x in FIX[name + "/params"])
     sv, u0 = (float(x) for x in FIX[name + "/fparams"])
@@ -50,7 +50,7 @@ def fixture_solver(name, impl_override=None):
                les=int(case in po.LES_CASES), cte_s=po.CTE_S_REF,
                trick1=int(case in po.TRICK1_CASES),
                forcing_y=int(case in po.FORCING_Y_CASES),
-               noshift=int(case in po.NOSHIFT_CASES))
+               noshift=int(case in po.NOSHIFT_CASES), **kw)
     return s, ns
 
 
@@ -104,6 +104,27 @@ def test_reference_fixture_run(name):
     got, ref = s.raw(), FIX[name + "/pops"]
     ok = same_bits(got[:, 1:-1, 1:-1], ref[:, 1:-1, 1:-1], s.prec)
     assert ok.all(), f"{int((~ok).sum())} differing elements"
+
+
+NS_FOLDED = [v for v in VARIANTS
+             if v.split("_")[1] == "FUSED" and v.split("_")[2].split("@")[0] in po.NOSHIFT_CASES
+             and po.CASE[v.split("_")[2].split("@")[0]][0] != "step"]
+
+
+@pytest.mark.parametrize("tb", [1, 2])
+@pytest.mark.parametrize("name", NS_FOLDED)
+def test_noshift_folded_kernels(name, tb):
+    """NOSHIFT (whole populations, SRC/col_MC.F90:70-74, the cte1 = 1
+    inflow/outflow equilibria of SRC/bcond.inflow.F90:48-52) through the
+    folded kernels, one step (tblock 1) and two steps (tblock 2) per pass:
+    interior equals the reference's NOSHIFT build"""
+    print(f"case {name} noshift tblock={tb}", flush=True)
+    s, ns = fixture_solver(name, tblock=tb)
+    s.run(ns)
+    got, ref = s.raw(), FIX[name + "/pops"]
+    ok = same_bits(got[:, 1:-1, 1:-1], ref[:, 1:-1, 1:-1], s.prec)
+    assert ok.all(), f"{int((~ok).sum())} differing elements"
+    s.close()
 
 
 CASES = [
