@@ -370,9 +370,11 @@ inline bool use_tb(const bgk_ctx *c)
 }
 
 // Strip blocks of height H that must apply the obstacle rule: those whose
-// updated nodes (their strip, its rim columns, one row above and below)
-// reach the obstacle's bounding box grown by one node, where fix_pulls
-// applies it -- with one more block on each side as a margin.  Along a
+// updated nodes reach the obstacle's bounding box grown by one node, where
+// fix_pulls applies it.  Strip bx computes step t+1 on columns bx*TW ..
+// bx*TW+TW+1 (its strip and rim columns), so it reaches the grown columns
+// imin-1..imax+1 exactly when (imin-3)/TW <= bx <= (imax+1)/TW (imin >= 2);
+// block row by computes rows by*H .. by*H+H+1, likewise.  Along a
 // periodic direction the first and last blocks also update wrapped nodes
 // of the far edge (their rim or halo row): when the grown box reaches an
 // edge (the obstacle touches the second or second-to-last row or column),
@@ -382,8 +384,8 @@ inline int4 tb_box(const bgk_ctx *c, int H, int TW, bool xper, bool yper)
 {
     if (!(c->p.obstacle && c->nobs > 0)) return make_int4(0, -1, 0, -1);
     const int strips = (c->l + TW - 1) / TW, rows = (c->m + H - 1) / H;
-    int4 b = make_int4(max(0, (c->imin - 3) / TW - 1), min(strips - 1, (c->imax + 1) / TW + 1),
-                       max(0, (c->jmin - 3) / H - 1), min(rows - 1, (c->jmax + 1) / H + 1));
+    int4 b = make_int4(max(0, (c->imin - 3) / TW), min(strips - 1, (c->imax + 1) / TW),
+                       max(0, (c->jmin - 3) / H), min(rows - 1, (c->jmax + 1) / H));
     if (xper && (c->imin - 1 <= 1 || c->imax + 1 >= c->l)) {
         b.x = 0;
         b.y = strips - 1;
